@@ -1,0 +1,237 @@
+"""The CPU oracle against the reference's own outputs.
+
+Fixtures in tests/golden/reference_vectors.npz come from the reference code
+compiled from its sources (tests/golden/make_golden.py); where the compiled
+reference (oracle/_ref) is present the same checks also run live.
+"""
+import ctypes
+import json
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+from conftest import REPO
+
+CASES = [xx + 4 * yy + 16 * zz for zz in range(3) for yy in range(3) for xx in range(3)]
+live_ref = pytest.mark.skipif(not po.ref_available(), reason="oracle/_ref not built (needs the reference checkout)")
+
+
+def _eval_table(entries, L, x):
+    out = np.zeros(5)
+    for e in entries:
+        if len(e) == 3:
+            out[e[0]] += e[1] * x[e[2]]
+        else:
+            out[e[0]] += e[1] * x[e[2]] * x[e[3]]
+    return L * out
+
+
+@pytest.fixture(scope="module")
+def tables():
+    with open(po.TABLES) as fh:
+        return json.load(fh)["tables"]
+
+
+def test_stencil_tables_reproduce_reference_outputs(golden, tables):
+    X = golden["stencil_x"]
+    for k in (2, 3, 4, 6):
+        for c in ["bulk"] + [str(c) for c in CASES]:
+            ref = golden["stencil_L%d_%s" % (k, c)]
+            for i in range(len(X)):
+                got = _eval_table(tables["L%d" % k][c], 1.7, X[i])
+                np.testing.assert_allclose(got, ref[i], rtol=1e-13, atol=1e-12 * max(1, np.abs(ref[i]).max()),
+                                           err_msg="L%d case %s" % (k, c))
+
+
+@live_ref
+def test_stencil_tables_match_live_reference(tables):
+    ref = po.ref_lib()
+    rng = np.random.default_rng(99)
+    for k in (2, 3, 4, 6):
+        for c in [-1] + CASES:
+            x = rng.standard_normal(125)
+            out = np.zeros(5)
+            ref.qmref_lk(k, 0.9, c, po.dptr(x[:5]), po.dptr(x[5:35]), po.dptr(x[35:]), po.dptr(out))
+            got = _eval_table(tables["L%d" % k]["bulk" if c < 0 else str(c)], 0.9, x)
+            np.testing.assert_allclose(got, out, rtol=1e-13, atol=1e-12 * max(1, np.abs(out).max()))
+
+
+def test_boundary_case_code_covers_27_cases(tables):
+    for k in (2, 3, 4, 6):
+        assert sorted(int(c) for c in tables["L%d" % k] if c != "bulk") == sorted(CASES)
+        # the bulk form is the no-object case (lcForces.h:90-97 vs :66-80)
+        assert tables["L%d" % k]["bulk"] == tables["L%d" % k]["0"]
+
+
+def test_phase_parts(golden):
+    lib = po.oracle_lib()
+    lib.orc_phase_parts.argtypes = [po._D, po._D, po._D]
+    for q, g, t in zip(golden["phase_q"], golden["phase_grad"], golden["phase_tr"]):
+        q = np.ascontiguousarray(q)
+        og, ot = np.zeros(15), np.zeros(3)
+        lib.orc_phase_parts(po.dptr(q), po.dptr(og), po.dptr(ot))
+        np.testing.assert_allclose(og, g, rtol=1e-13, atol=1e-15)
+        np.testing.assert_allclose(ot, t, rtol=1e-13, atol=1e-15)
+
+
+def test_l1_stencils(golden):
+    lib = po.oracle_lib()
+    lib.orc_l1_site.argtypes = [ctypes.c_double, ctypes.c_int, po._D, po._D, po._I, po._D]
+    X, T = golden["stencil_x"], golden["l1_types"]
+    for i in range(len(X)):
+        x, t = np.ascontiguousarray(X[i]), np.ascontiguousarray(T[i])
+        out = np.zeros(5)
+        lib.orc_l1_site(2.32, 0, po.dptr(x[:5]), po.dptr(x[5:35]), po.iptr(t), po.dptr(out))
+        np.testing.assert_allclose(out, golden["l1_bulk"][i], rtol=1e-13, atol=1e-13)
+        lib.orc_l1_site(2.32, 1, po.dptr(x[:5]), po.dptr(x[5:35]), po.iptr(t), po.dptr(out))
+        np.testing.assert_allclose(out, golden["l1_boundary"][i], rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("kind,bt", [("h", 0), ("p", 1)])
+def test_anchoring(golden, kind, bt):
+    lib = po.oracle_lib()
+    lib.orc_anchoring_force.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, po._D, po._D, po._D]
+    lib.orc_anchoring_energy.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, po._D, po._D]
+    lib.orc_anchoring_energy.restype = ctypes.c_double
+    for i, q in enumerate(golden["anch_q"]):
+        q = np.ascontiguousarray(q)
+        qb = np.ascontiguousarray(golden["anch_qb_" + kind][i])
+        f = np.zeros(5)
+        lib.orc_anchoring_force(bt, 5.3, 0.53, po.dptr(q), po.dptr(qb), po.dptr(f))
+        ref = golden["anch_force_" + kind][i]
+        np.testing.assert_allclose(f, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+        e = lib.orc_anchoring_energy(bt, 5.3, 0.53, po.dptr(q), po.dptr(qb))
+        np.testing.assert_allclose(e, golden["anch_energy_" + kind][i], rtol=1e-12)
+
+
+def test_eigenvalues(golden):
+    lib = po.oracle_lib()
+    for q, ev in zip(golden["eig_q"], golden["eig_ev"]):
+        out = np.zeros(3)
+        lib.orc_eigenvalues(po.dptr(np.ascontiguousarray(q)), po.dptr(out))
+        np.testing.assert_allclose(out, ev, rtol=1e-12, atol=1e-14)
+
+
+def test_lattice_init_types_defects(golden, phase):
+    A, B, C, S0 = phase
+    shape = tuple(golden["small_shape"])
+    o = po.OracleSystem(*shape)
+    assert np.array_equal(o.neighbors(), golden["small_neighbors"])
+    o.random_init(1234, S0)
+    assert np.array_equal(o.q(), golden["small_init_q"]), "seeded initial Q must be bit-exact"
+    o.wall(2, 0, 1, 5.3, S0)
+    o.sphere((3.0, 2.5, 3.5), 1.8, 0, 5.3, S0)
+    assert np.array_equal(o.types(), golden["small_types"])
+    d = o.defects(0)
+    lc = o.types() <= 0
+    np.testing.assert_allclose(d[lc], golden["small_defects"][lc], rtol=1e-13, atol=1e-15)
+
+
+def test_uniform_stream_is_libstdcxx_generate_canonical(golden):
+    """getRealUniform = uniform_real_distribution<double> over mt19937: two 32-bit
+    draws per double, (a + b 2^32) / 2^64 (libstdc++ generate_canonical).  numpy's
+    legacy RandomState(seed) seeds MT19937 with the same init_genrand."""
+    bg = np.random.RandomState(1234)._bit_generator
+    raw = bg.random_raw(128).astype(np.float64)
+    u = (raw[0::2] + raw[1::2] * 2.0 ** 32) / 2.0 ** 64
+    np.testing.assert_array_equal(u, golden["uniform_1234"])
+
+
+def test_mt19937_known_answer():
+    # The C++ standard fixes the 10000th output of a default-seeded mt19937.
+    bg = np.random.RandomState(5489)._bit_generator
+    assert int(bg.random_raw(10000)[-1]) == 4123659995
+
+
+def _scene(phase, params, sem, ef=None):
+    A, B, C, S0 = phase
+    o = po.OracleSystem(6, 5, 7)
+    o.random_init(1234, S0)
+    o.wall(2, 0, 1, 5.3, S0)
+    o.sphere((3.0, 2.5, 3.5), 1.8, 0, 5.3, S0)
+    o.set_params(A, B, C, *params, semantics=sem)
+    if ef is not None:
+        o.set_efield(ef, 1.0, 1.0, 1.0)
+    return o
+
+
+@pytest.mark.parametrize("name,params,sem,ef", [
+    ("one", (2.32, 0.0, 0.0, 0.0, 0.0), 0, None),
+    ("multi", (2.32, 0.5, 0.3, 0.0, 0.0), 0, None),
+    ("all", (2.32, 1.0, 0.5, 0.2, 0.3), 0, (0.0, 0.0, 0.5)),
+    ("multi_gpu", (2.32, 0.5, 0.3, 0.0, 0.0), 1, None),
+])
+def test_fire_trajectory_bit_exact(golden, phase, name, params, sem, ef):
+    o = _scene(phase, params, sem, ef)
+    it, tr = o.fire(max_iter=40)
+    assert it == 40
+    assert np.array_equal(tr, golden["fire_%s_trace" % name])
+    assert np.array_equal(o.q(), golden["fire_%s_q" % name])
+
+
+def test_adam_nesterov_lolbfgs_bit_exact(golden, phase):
+    p = (2.32, 0.5, 0.3, 0.0, 0.0)
+    o = _scene(phase, p, 0)
+    it, tr = o.adam(dt=1e-3, max_iter=30)
+    assert np.array_equal(tr, golden["adam_trace"]) and np.array_equal(o.q(), golden["adam_q"])
+    o = _scene(phase, p, 0)
+    it, tr = o.nesterov(dt=1e-2, mu=0.5, scheduled=True, max_iter=30)
+    assert np.array_equal(tr, golden["nesterov_trace"]) and np.array_equal(o.q(), golden["nesterov_q"])
+    o = _scene(phase, p, 0)
+    it, tr = o.lolbfgs(m=8, dt=1e-2, max_iter=30)
+    assert np.array_equal(tr, golden["lolbfgs_trace"]) and np.array_equal(o.q(), golden["lolbfgs_q"])
+
+
+@live_ref
+def test_fire_minimize_loop_matches_reference_minimize(phase):
+    """The reference's own minimize() (stopping rule included) against the oracle."""
+    A, B, C, S0 = phase
+    o = _scene(phase, (2.32, 0.0, 0.0, 0.0, 0.0), 0)
+    r = po.RefModel(6, 5, 7)
+    r.set_q(o.q())
+    from golden.make_golden import sphere_sites  # noqa: E402
+    wall = np.array([x + 6 * (y + 5 * 0) for x in range(6) for y in range(5)], np.int32)
+    r.add_object(wall, 1, 5.3, S0)
+    r.add_object(sphere_sites((6, 5, 7), (3.0, 2.5, 3.5), 1.8), 0, 5.3, S0)
+    fn, ctx = o.force_callback()
+    fmax = np.zeros(1)
+    lib = po.ref_lib()
+    its = lib.qmref_fire_minimize(r.h, fn, ctx, 5e-4, 0.99, 0.05, 1.1, 0.95, 0.9, 4, 0.05, 0.0, 5000, po.dptr(fmax))
+    o2 = _scene(phase, (2.32, 0.0, 0.0, 0.0, 0.0), 0)
+    it2, tr = o2.fire(fc=0.05, max_iter=5000)
+    assert its == it2 and tr[-1, 0] == fmax[0] and tr[-1, 0] <= 0.05
+    assert np.array_equal(r.q(), o2.q())
+
+
+REF_ENERGY_SRC = "/root/reference/src/forces/landauDeGennesLC.cpp"
+
+
+@pytest.mark.skipif(not os.path.exists(REF_ENERGY_SRC), reason="reference checkout absent")
+def test_distortion_energy_matches_reference_polynomials():
+    """The oracle writes the L1..L6 energy densities from tensor invariants; the
+    reference spells them out as polynomials (landauDeGennesLC.cpp:268-335).
+    Evaluate the reference's own expressions, read from its source text, on
+    random inputs and compare term by term."""
+    import re
+    src = open(REF_ENERGY_SRC).read()
+    body = src[src.index("void landauDeGennesLC::computeEnergyCPU"):src.index("#ifdef ENABLE_CUDA\n\nvoid landauDeGennesLC::computeForceGPU")]
+    exprs = {}
+    for name in ("L1", "L2", "L3", "L4", "L6"):
+        m = re.search(r"if\(%s != 0 \)\s*\{\s*distortionEnergyAtSite \+= (.*?);\s*\};" % name, body, re.S)
+        assert m, name
+        e = m.group(1)
+        e = re.sub(r"firstDerivative([XYZ])\[(\d)\]", r"d\1[\2]", e).replace("qCurrent", "q")
+        exprs[name] = compile(e, name, "eval")
+    lib = po.oracle_lib()
+    lib.orc_distortion_density.argtypes = [po._D] * 5
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        q, dX, dY, dZ = (np.ascontiguousarray(rng.standard_normal(5)) for _ in range(4))
+        e = np.zeros(5)
+        lib.orc_distortion_density(po.dptr(q), po.dptr(dX), po.dptr(dY), po.dptr(dZ), po.dptr(e))
+        env = {"q": q, "dX": dX, "dY": dY, "dZ": dZ}
+        for i, name in enumerate(("L1", "L2", "L3", "L4", "L6")):
+            ref = eval(exprs[name], {name: 1.0}, env)
+            np.testing.assert_allclose(e[i], ref, rtol=1e-12, atol=1e-12, err_msg=name)
