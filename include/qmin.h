@@ -1,0 +1,195 @@
+/*
+ * qmin: MI355X (gfx950) lattice Landau-de Gennes energy minimization.
+ *
+ * C ABI of the HIP library libqmin.so.  Plain pointers and sizes only; every
+ * function returns QM_OK (0) or a negative QM_E* code, with a message from
+ * qm_last_error().  Host arrays use the reference's layouts: Q, v, F are AoS
+ * double[N][5] (Qxx, Qxy, Qxz, Qyy, Qyz), site = x + Lx*(y + Ly*z), types are
+ * int32 (0 bulk, -1 next to an object, j >= 1 object j).  On the device the
+ * library keeps structure-of-arrays planes in FP64 or FP32.
+ *
+ * Each entry point names the reference interface it replaces (open-Qmin,
+ * paths relative to the reference checkout).  The C++ classes in
+ * open-qmin_amd/host/ wrap these calls under the reference's class and method
+ * names.
+ */
+#ifndef QMIN_H
+#define QMIN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    QM_OK = 0,
+    QM_EINVAL = -1,   /* bad argument */
+    QM_EHIP = -2,     /* HIP runtime error */
+    QM_ENOMEM = -3,   /* allocation failed */
+    QM_EIO = -4,      /* file error */
+    QM_ESTATE = -5,   /* call not valid in the current state */
+    QM_ENAN = -6      /* the minimizer produced NaN forces */
+};
+
+enum { QM_FP64 = 0, QM_FP32 = 1 };                     /* device storage and stencil precision */
+enum { QM_HOMEOTROPIC = 0, QM_DEGENERATE_PLANAR = 1 }; /* src/model/latticeBoundaries.h:8 */
+enum { QM_SEMANTICS_REF_CPU = 0, QM_SEMANTICS_REF_GPU = 1 };
+enum { QM_FIELD_Q = 0, QM_FIELD_V = 1, QM_FIELD_F = 2 };
+
+typedef struct qm_lattice qm_lattice;
+typedef struct qm_ldg qm_ldg;
+typedef struct qm_fire qm_fire;
+typedef struct qm_adam qm_adam;
+typedef struct qm_nesterov qm_nesterov;
+typedef struct qm_lolbfgs qm_lolbfgs;
+
+const char* qm_last_error(void);
+int qm_device_count(int* count);
+
+/* ---- lattice model --------------------------------------------------------
+ * Replaces multirankQTensorLatticeModel(lx,ly,lz,...) on one rank without halos
+ * (src/model/multirankQTensorLatticeModel.cpp:8-50) with its cubicLattice and
+ * qTensorLatticeModel bases (src/model/cubicLattice.cpp:22-36,
+ * src/model/qTensorLatticeModel.h:25-62).  One lattice is one periodic domain
+ * resident in HBM. */
+int qm_lattice_create(int lx, int ly, int lz, int precision, qm_lattice** out);
+int qm_lattice_destroy(qm_lattice* lat);
+int qm_lattice_dims(const qm_lattice* lat, int* lx, int* ly, int* lz, int* precision);
+/* AoS host <-> device state; which = QM_FIELD_Q/V/F (simpleModel::returnPositions/
+ * returnVelocities/returnForces, src/model/simpleModel.h:63-73) */
+int qm_lattice_set_field(qm_lattice* lat, int which, const double* aos);
+int qm_lattice_get_field(qm_lattice* lat, int which, double* aos);
+/* qTensorLatticeModel::setNematicQTensorRandomly after noiseSource::setReproducibleSeed
+ * (src/model/qTensorLatticeModel.cpp:134-177, src/utilities/noiseSource.cpp:38-47) */
+int qm_lattice_random_init(qm_lattice* lat, int seed, double S0);
+/* multirankQTensorLatticeModel::setUniformDirectors-like uniform texture from a director */
+int qm_lattice_uniform_init(qm_lattice* lat, double nx, double ny, double nz, double S0);
+/* multirankSimulation::createMultirankBoundaryObject + cubicLattice::createBoundaryObject
+ * (src/simulation/multirankSimulationBoundaries.cpp:512-543, src/model/cubicLattice.cpp:267-318);
+ * xyz is [n][3] (wrapped periodically), qb is [n][5]; returns the object index via *obj */
+int qm_lattice_add_object(qm_lattice* lat, const int32_t* xyz, int64_t n, const double* qb, int btype, double W,
+                          double S0, int* obj);
+/* multirankSimulation::createWall (multirankSimulationBoundaries.cpp:101-167) */
+int qm_lattice_wall(qm_lattice* lat, int xyz, int plane, int btype, double W, double S0, int* obj);
+/* multirankSimulation::createSphericalColloid (multirankSimulationBoundaries.cpp:169-215) */
+int qm_lattice_sphere(qm_lattice* lat, double cx, double cy, double cz, double radius, int btype, double W,
+                      double S0, int* obj);
+/* multirankSimulation::createBoundaryFromFile (multirankSimulationBoundaries.cpp:41-96) */
+int qm_lattice_boundary_file(qm_lattice* lat, const char* path);
+int qm_lattice_get_types(const qm_lattice* lat, int32_t* types);
+/* neighbour table in the reference order {x-,x+,y-,y+,z-,z+}, int32[N][6]
+ * (cubicLattice::fillNeighborLists, src/model/cubicLattice.cpp:143-166) */
+int qm_lattice_get_neighbors(const qm_lattice* lat, int32_t* nbrs);
+/* updater::getNTotal (src/updaters/baseUpdater.cpp:10-28): number of sites with t <= 0 */
+int qm_lattice_n_active(const qm_lattice* lat, int64_t* n);
+int qm_lattice_n_boundary(const qm_lattice* lat, int64_t* n);
+/* qTensorLatticeModel::computeDefectMeasures (src/model/qTensorLatticeModel.cpp:95-133):
+ * kind 0 largest eigenvalue, 1 det Q, 2 TrQ2^3 - 54 det^2; out is double[N] */
+int qm_lattice_defects(qm_lattice* lat, int kind, double* out);
+/* multirankSimulation::saveState / loadState text format, one rank
+ * (src/simulation/multirankSimulation.cpp:573-665): lines "x y z Qxx Qxy Qxz Qyy Qyz type Smax";
+ * path is the full file name */
+int qm_lattice_save_state(qm_lattice* lat, const char* path);
+int qm_lattice_load_state(qm_lattice* lat, const char* path);
+/* copy the device state Q of one lattice into another of the same size */
+int qm_lattice_copy_q(qm_lattice* dst, const qm_lattice* src);
+
+/* ---- force computer: landauDeGennesLC (src/forces/landauDeGennesLC.h:12-173) */
+int qm_ldg_create(qm_ldg** out);
+int qm_ldg_destroy(qm_ldg* f);
+/* setPhaseConstants(A,B,C) (landauDeGennesLC.h:34): energy A/2 TrQ^2 + B/3 TrQ^3 + C/4 (TrQ^2)^2 */
+int qm_ldg_set_phase(qm_ldg* f, double A, double B, double C);
+/* setElasticConstants(L1,L2,L3,L4,L6) + setNumberOfConstants (landauDeGennesLC.h:35-36);
+ * multi = 0 selects the one-constant (L1) path */
+int qm_ldg_set_elastic(qm_ldg* f, double L1, double L2, double L3, double L4, double L6, int multi);
+/* setEField(E, eps, eps0, deltaEps) / setHField(H, chi, mu0, deltaChi) (landauDeGennesLC.h:76-91);
+ * on = 0 removes the field */
+int qm_ldg_set_efield(qm_ldg* f, int on, double Ex, double Ey, double Ez, double eps, double eps0, double deps);
+int qm_ldg_set_hfield(qm_ldg* f, int on, double Hx, double Hy, double Hz, double chi, double mu0, double dchi);
+/* QM_SEMANTICS_REF_GPU (default): one metric correction after all terms, as the
+ * reference GPU path (landauDeGennesLC.cpp:128-134, 357-398).
+ * QM_SEMANTICS_REF_CPU: the reference CPU path's two passes, which correct bulk
+ * sites twice (multirankSimulation.cpp:462-466). */
+int qm_ldg_set_semantics(qm_ldg* f, int semantics);
+/* force::computeForces (src/forces/baseForce.h:15-77): F = forces(Q) into the lattice */
+int qm_ldg_compute_forces(qm_ldg* f, qm_lattice* lat);
+/* landauDeGennesLC::computeEnergy (landauDeGennesLC.cpp:204-353): total energy and
+ * components {phase, distortion, anchoring, E field, H field}; the reference's
+ * computePotentialEnergy reports total / n_active (multirankSimulation.cpp:475-490) */
+int qm_ldg_compute_energy(qm_ldg* f, qm_lattice* lat, double* total, double* components5);
+
+/* ---- FIRE: energyMinimizerFIRE (src/updaters/energyMinimizerFIRE.h:20-110) */
+typedef struct {
+    int iterations;      /* cumulative, as updater::iterations */
+    int nPositive;       /* NSinceNegativePower */
+    int converged;       /* last minimize stopped on the force tolerance */
+    int nanDetected;     /* forceMax became NaN (the reference would report convergence) */
+    double forceMax;     /* sqrt(sum dotVec(F,F)) / nTotal (energyMinimizerFIRE.cpp:110) */
+    double power, alpha, deltaT, scaling;
+} qm_fire_state;
+
+int qm_fire_create(qm_fire** out);
+int qm_fire_destroy(qm_fire* m);
+/* setFIREParameters(dt, alphaStart, dtMax, dtInc, dtDec, alphaDec, nMin, forceCutoff, alphaMin)
+ * (energyMinimizerFIRE.cpp:172-183); also resets alpha and dtMin = dt/100 */
+int qm_fire_set_parameters(qm_fire* m, double dt, double alphaStart, double dtMax, double dtInc, double dtDec,
+                           double alphaDec, int nMin, double forceCutoff, double alphaMin);
+/* updater::setMaximumIterations (baseUpdater.h:77) */
+int qm_fire_set_max_iterations(qm_fire* m, int maxIterations);
+/* energyMinimizerFIRE::minimize (energyMinimizerFIRE.cpp:151-170) on the device:
+ * forces, then fused velocity-Verlet + force + FIRE iterations until the tolerance
+ * or the iteration cap.  The velocity field persists in the lattice. */
+int qm_fire_minimize(qm_fire* m, qm_ldg* f, qm_lattice* lat);
+/* run exactly n iterations (no tolerance test): throughput measurement */
+int qm_fire_run_iterations(qm_fire* m, qm_ldg* f, qm_lattice* lat, int n);
+int qm_fire_get_state(const qm_fire* m, qm_fire_state* out);
+/* per-iteration trace {forceMax, power, alpha, deltaT, scaling, nPositive}: capacity in
+ * iterations (0 disables); qm_fire_get_trace copies up to cap rows, returns the count in *n */
+int qm_fire_set_trace(qm_fire* m, int capacity);
+int qm_fire_get_trace(qm_fire* m, double* rows6, int cap, int* n);
+
+/* ---- Adam: energyMinimizerAdam (src/updaters/energyMinimizerAdam.h:7-50) */
+int qm_adam_create(qm_adam** out);
+int qm_adam_destroy(qm_adam* m);
+/* setAdamParameters(b1, b2, eps, dt, fc) (energyMinimizerAdam.h:16-25) */
+int qm_adam_set_parameters(qm_adam* m, double beta1, double beta2, double eps, double dt, double forceCutoff);
+int qm_adam_set_max_iterations(qm_adam* m, int maxIterations);
+/* energyMinimizerAdam::minimize (energyMinimizerAdam.cpp:56-75); forceMax = sqrt(F.F)/N */
+int qm_adam_minimize(qm_adam* m, qm_ldg* f, qm_lattice* lat);
+int qm_adam_get_state(const qm_adam* m, int* iterations, double* forceMax);
+
+/* ---- Nesterov: energyMinimizerNesterovAG (src/updaters/energyMinimizerNesterovAG.h:8-61) */
+int qm_nesterov_create(qm_nesterov** out);
+int qm_nesterov_destroy(qm_nesterov* m);
+/* setNesterovAGParameters(dt, mu, fc) + scheduledMomentum (energyMinimizerNesterovAG.h:22-31) */
+int qm_nesterov_set_parameters(qm_nesterov* m, double dt, double mu, double forceCutoff, int scheduledMomentum);
+int qm_nesterov_set_max_iterations(qm_nesterov* m, int maxIterations);
+/* energyMinimizerNesterovAG::minimize (energyMinimizerNesterovAG.cpp:94-120) */
+int qm_nesterov_minimize(qm_nesterov* m, qm_ldg* f, qm_lattice* lat);
+int qm_nesterov_get_state(const qm_nesterov* m, int* iterations, double* forceMax);
+
+/* ---- online L-BFGS: energyMinimizerLoLBFGS (src/updaters/energyMinimizerLoLBFGS.h:13-110) */
+int qm_lolbfgs_create(qm_lolbfgs** out);
+int qm_lolbfgs_destroy(qm_lolbfgs* m);
+/* setLoLBFGSParameters(m, dt, c, fc, tau) + scheduledMomentum (energyMinimizerLoLBFGS.h:29-45);
+ * the memory never shrinks below 5, as in the reference */
+int qm_lolbfgs_set_parameters(qm_lolbfgs* m, int memory, double dt, double c, double forceCutoff, double tau,
+                              int scheduledMomentum);
+int qm_lolbfgs_set_max_iterations(qm_lolbfgs* m, int maxIterations);
+/* energyMinimizerLoLBFGS::minimize (energyMinimizerLoLBFGS.cpp:195-226) */
+int qm_lolbfgs_minimize(qm_lolbfgs* m, qm_ldg* f, qm_lattice* lat);
+int qm_lolbfgs_get_state(const qm_lolbfgs* m, int* iterations, double* forceMax);
+
+/* ---- kernel timing (HIP events on the library's stream) -------------------
+ * When enabled, every launch of the two FIRE kernels is bracketed by events;
+ * qm_timing_get reports launches and total milliseconds per kernel:
+ * kernel 0 = fused lattice pass, 1 = boundary-site pass (with the FIRE scalar step). */
+int qm_timing_enable(int on);
+int qm_timing_reset(void);
+int qm_timing_get(int kernel, int64_t* launches, double* total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QMIN_H */

@@ -1,0 +1,861 @@
+// gfx950 kernels of the lattice Landau-de Gennes minimizer.
+//
+// Fused lattice pass (march_kernel): one work-group owns a 32 x 8 column of
+// sites and marches it along a z chunk.  Each step stages one halo-padded
+// plane of the new state into a 3-plane LDS ring: in FIRE mode the pending
+// FIRE velocity mix, the velocity-Verlet drift and the first half kick are
+// applied while loading (Q' = Q + dt v + dt^2/2 F, v' = v + dt/2 F), so the
+// force at Q' is computed from LDS without another pass over HBM.  Every
+// site's force needs only its 3x3x3 neighbourhood minus the corners: the
+// reference's stencils use neighbour derivatives transverse to the neighbour
+// direction only (src/forces/lcForces.h:66-1043).  The second half kick and
+// the FP64 partial sums of F.F, F.v, v.v (QT metric, inc/functions.h:140-147)
+// close the step.  Sites next to an object (t < 0) are left to the boundary
+// pass, which keeps the rarely-needed 27-case boundary stencils out of this
+// kernel's register budget.
+//
+// Boundary pass (boundary_kernel): one thread per site of the compacted t < 0
+// list; 27-case stencils, anchoring, fields and metric; then the last
+// work-group to arrive sums all partials in a fixed order and advances the FIRE
+// scalars on the device (energyMinimizerFIRE.cpp:77-145): no host round trip.
+#include "qmin_device.h"
+#include "ldg_math.h"
+#include "gen/ldg_stencils.inc"
+
+namespace qmin {
+
+__device__ __forceinline__ int wrapi(int v, int m) {
+    v %= m;
+    return v < 0 ? v + m : v;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Block-wide sum of three doubles in a fixed order; result valid in thread 0.
+template <int NT>
+__device__ __forceinline__ void block_sum3(double& a, double& b, double& c, double (*red)[3]) {
+    a = wave_sum(a);
+    b = wave_sum(b);
+    c = wave_sum(c);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) {
+        red[w][0] = a;
+        red[w][1] = b;
+        red[w][2] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+        for (int i = 0; i < NT / 64; ++i) {
+            s0 += red[i][0];
+            s1 += red[i][1];
+            s2 += red[i][2];
+        }
+        a = s0;
+        b = s1;
+        c = s2;
+    }
+}
+
+// Derivative at neighbour n along one transverse axis, with the reference's
+// rules (landauDeGennesLCOtherForces.cpp:15-145): zero on objects, central on
+// t == 0 or when neither side is an object, otherwise one-sided with an object
+// above taking precedence.
+template <typename R>
+__device__ __forceinline__ void nbr_derivative(int tn, int tu, int td, const R* nq, const R* qu, const R* qd, R* out) {
+    if (tn > 0) {
+#pragma unroll
+        for (int c = 0; c < 5; ++c) out[c] = R(0);
+    } else if (tn == 0 || (td <= 0 && tu <= 0)) {
+#pragma unroll
+        for (int c = 0; c < 5; ++c) out[c] = R(0.5) * (qu[c] - qd[c]);
+    } else if (tu > 0) {
+#pragma unroll
+        for (int c = 0; c < 5; ++c) out[c] = nq[c] - qd[c];
+    } else {
+#pragma unroll
+        for (int c = 0; c < 5; ++c) out[c] = qu[c] - nq[c];
+    }
+}
+
+template <typename R>
+struct Tile {
+    R (*q)[5][kHP];
+    int8_t (*t)[kHP];
+    __device__ __forceinline__ void load(int s, int h, R* v) const {
+#pragma unroll
+        for (int c = 0; c < 5; ++c) v[c] = q[s][c][h];
+    }
+};
+
+// Neighbour d of the centre (slot sc, index h) and, for each of its two
+// transverse axes A < B, the positions of n + e_axis and n - e_axis.
+struct NbrGeom {
+    int sn, hn;
+    int suA, huA, sdA, hdA;
+    int suB, huB, sdB, hdB;
+};
+
+__device__ __forceinline__ NbrGeom nbr_geom(int d, int sm, int sc, int sp, int h) {
+    NbrGeom g;
+    switch (d) {
+        case 0:
+        case 1: {
+            const int hn = d == 0 ? h - 1 : h + 1;
+            g = {sc, hn, sc, hn + kHX, sc, hn - kHX, sp, hn, sm, hn};
+            break;
+        }
+        case 2:
+        case 3: {
+            const int hn = d == 2 ? h - kHX : h + kHX;
+            g = {sc, hn, sc, hn + 1, sc, hn - 1, sp, hn, sm, hn};
+            break;
+        }
+        default: {
+            const int sn = d == 4 ? sm : sp;
+            g = {sn, h, sn, h + 1, sn, h - 1, sn, h + kHX, sn, h - kHX};
+            break;
+        }
+    }
+    return g;
+}
+
+template <typename R, int D>
+__device__ __forceinline__ void bulk_lin_nbr(const R* nq, const R* dA, const R* dB, R (&acc)[3][5]) {
+    if (D == 0) ldg_bulk_lin_n0(nq, dA, dB, acc);
+    if (D == 1) ldg_bulk_lin_n1(nq, dA, dB, acc);
+    if (D == 2) ldg_bulk_lin_n2(nq, dA, dB, acc);
+    if (D == 3) ldg_bulk_lin_n3(nq, dA, dB, acc);
+    if (D == 4) ldg_bulk_lin_n4(nq, dA, dB, acc);
+    if (D == 5) ldg_bulk_lin_n5(nq, dA, dB, acc);
+}
+
+template <typename R, int TERMS, int D>
+__device__ __forceinline__ void bulk_neighbour(const Tile<R>& T, int sm, int sc, int sp, int h, R (&acc)[3][5],
+                                               R* x) {
+    const NbrGeom g = nbr_geom(D, sm, sc, sp, h);
+    R nq[5], qu[5], qd[5], dA[5], dB[5];
+    T.load(g.sn, g.hn, nq);
+    const int tn = T.t[g.sn][g.hn];
+    T.load(g.suA, g.huA, qu);
+    T.load(g.sdA, g.hdA, qd);
+    nbr_derivative(tn, int(T.t[g.suA][g.huA]), int(T.t[g.sdA][g.hdA]), nq, qu, qd, dA);
+    T.load(g.suB, g.huB, qu);
+    T.load(g.sdB, g.hdB, qd);
+    nbr_derivative(tn, int(T.t[g.suB][g.huB]), int(T.t[g.sdB][g.hdB]), nq, qu, qd, dB);
+    bulk_lin_nbr<R, D>(nq, dA, dB, acc);
+    if constexpr (TERMS == kMultiL6) {
+        // x layout of the stencil tables: q[5], neighbour Q [6][5], derivatives [6][3][5]
+        const int axis = D / 2, a = axis == 0 ? 1 : 0, b = axis == 2 ? 1 : 2;
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            x[5 + 5 * D + c] = nq[c];
+            x[35 + 15 * D + 5 * a + c] = dA[c];
+            x[35 + 15 * D + 5 * b + c] = dB[c];
+            x[35 + 15 * D + 5 * axis + c] = R(0);
+        }
+    }
+}
+
+// Raw force (phase - distortion) of a t == 0 site at LDS centre h of slot sc.
+template <typename R, int TERMS>
+__device__ __forceinline__ void bulk_raw_force(const ForceConst& fc, const Tile<R>& T, int sm, int sc, int sp, int h,
+                                               const R* q, R* F) {
+#pragma unroll
+    for (int c = 0; c < 5; ++c) F[c] = R(0);
+    phase_force(R(fc.a), R(fc.b), R(fc.c), q, F);
+    R spatial[5];
+    {
+        R nb[5], lap[5];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) lap[c] = R(6) * q[c];
+        T.load(sc, h - 1, nb);
+#pragma unroll
+        for (int c = 0; c < 5; ++c) lap[c] -= nb[c];
+        T.load(sc, h + 1, nb);
+#pragma unroll
+        for (int c = 0; c < 5; ++c) lap[c] -= nb[c];
+        T.load(sc, h - kHX, nb);
+#pragma unroll
+        for (int c = 0; c < 5; ++c) lap[c] -= nb[c];
+        T.load(sc, h + kHX, nb);
+#pragma unroll
+        for (int c = 0; c < 5; ++c) lap[c] -= nb[c];
+        T.load(sm, h, nb);
+#pragma unroll
+        for (int c = 0; c < 5; ++c) lap[c] -= nb[c];
+        T.load(sp, h, nb);
+#pragma unroll
+        for (int c = 0; c < 5; ++c) spatial[c] = R(fc.L1) * (lap[c] - nb[c]);
+        l1_shape(spatial);
+    }
+    if constexpr (TERMS != kOneConstant) {
+        R acc[3][5];
+#pragma unroll
+        for (int t = 0; t < 3; ++t)
+#pragma unroll
+            for (int c = 0; c < 5; ++c) acc[t][c] = R(0);
+        R x[TERMS == kMultiL6 ? 125 : 1];
+        ldg_bulk_lin_q(q, acc);
+        bulk_neighbour<R, TERMS, 0>(T, sm, sc, sp, h, acc, x);
+        bulk_neighbour<R, TERMS, 1>(T, sm, sc, sp, h, acc, x);
+        bulk_neighbour<R, TERMS, 2>(T, sm, sc, sp, h, acc, x);
+        bulk_neighbour<R, TERMS, 3>(T, sm, sc, sp, h, acc, x);
+        bulk_neighbour<R, TERMS, 4>(T, sm, sc, sp, h, acc, x);
+        bulk_neighbour<R, TERMS, 5>(T, sm, sc, sp, h, acc, x);
+#pragma unroll
+        for (int c = 0; c < 5; ++c) spatial[c] += R(fc.L2) * acc[0][c];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) spatial[c] += R(fc.L3) * acc[1][c];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) spatial[c] += R(fc.L4) * acc[2][c];
+        if constexpr (TERMS == kMultiL6) {
+#pragma unroll
+            for (int c = 0; c < 5; ++c) x[c] = q[c];
+            R o6[5];
+            ldg_bulk_l6(x, o6);
+#pragma unroll
+            for (int c = 0; c < 5; ++c) spatial[c] += R(fc.L6) * o6[c];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 5; ++c) F[c] -= spatial[c];
+}
+
+template <typename R>
+__device__ __forceinline__ void finish_bulk(const ForceConst& fc, R* F) {
+    if (fc.semantics == 0) metric(F);
+    if (fc.hasField) {
+#pragma unroll
+        for (int c = 0; c < 5; ++c) F[c] += R(fc.field[c]);
+    }
+    metric(F);
+}
+
+template <typename R, int MODE, int TERMS>
+__global__ void __launch_bounds__(kThreads) march_kernel(MarchArgs<R> A) {
+    __shared__ R sQ[3][5][kHP];
+    __shared__ int8_t sT[3][kHP];
+    __shared__ double sRed[kThreads / 64][3];
+
+    Ctrl* ctrl = A.ctrl;
+    if (MODE == kFire && ctrl->done) return;
+    const int cur = ctrl->cur;
+    const StateSet<R> in = A.set[cur];
+    const StateSet<R> out = A.set[MODE == kFire ? cur ^ 1 : cur];
+    R dt = 0, hdt = 0, hdt2 = 0, ck = 0, cf = 0;
+    int mix = 0;  // 0 v = u, 1 v = ck u + cf F, 2 v = 0
+    if (MODE == kFire) {
+        const double dtd = ctrl->dt;
+        dt = R(dtd);
+        hdt = R(0.5 * dtd);
+        hdt2 = R(0.5 * dtd * dtd);
+        ck = R(ctrl->cKeep);
+        cf = R(ctrl->cForce);
+        mix = ctrl->reset ? 2 : (ctrl->identity ? 0 : 1);
+    }
+
+    // XCD-aware tile order: consecutive work-group ids go round-robin over the
+    // 8 XCDs; give each XCD a contiguous run of tiles so that tiles sharing
+    // halos sit in the same L2.
+    const int nb = A.nblocks, b = blockIdx.x;
+    const int qb = nb >> 3, rb = nb & 7, xcd = b & 7;
+    const int logical = xcd * qb + min(xcd, rb) + (b >> 3);
+    const int txi = logical % A.nbx;
+    const int rest = logical / A.nbx;
+    const int tyi = rest % A.nby;
+    const int zc = rest / A.nby;
+    const int z0 = zc * A.zchunk;
+    const int z1 = min(z0 + A.zchunk, A.lz);
+
+    const int lx = A.lx, ly = A.ly, lz = A.lz;
+    const long long n = A.n;
+    const long long plane = (long long)lx * ly;
+    const int tid = threadIdx.x;
+    const int ox = tid % kTX, oy = tid / kTX;
+    const int x = txi * kTX + ox, y = tyi * kTY + oy;
+    const bool active = x < lx && y < ly;
+    const int hOwn = (oy + 1) * kHX + (ox + 1);
+    const long long colOwn = wrapi(x, lx) + (long long)lx * wrapi(y, ly);
+    int hHalo = -1;
+    long long colHalo = 0;
+    if (tid < kHalo) {
+        const int k = tid;
+        if (k < kHX)
+            hHalo = k;
+        else if (k < 2 * kHX)
+            hHalo = (kHY - 1) * kHX + (k - kHX);
+        else if (k < 2 * kHX + (kHY - 2))
+            hHalo = (1 + k - 2 * kHX) * kHX;
+        else
+            hHalo = (1 + k - 2 * kHX - (kHY - 2)) * kHX + (kHX - 1);
+        const int hx = hHalo % kHX, hy = hHalo / kHX;
+        colHalo = wrapi(txi * kTX - 1 + hx, lx) + (long long)lx * wrapi(tyi * kTY - 1 + hy, ly);
+    }
+
+    const Tile<R> T{sQ, sT};
+
+    // Stage plane z into slot s; returns the owner's half-kicked velocity.
+    auto stage = [&](int z, int s, R* vh, int& tOwn) {
+        const long long zoff = plane * wrapi(z, lz);
+        {
+            const long long i = colOwn + zoff;
+            const int8_t t = A.types[i];
+            R q[5];
+#pragma unroll
+            for (int c = 0; c < 5; ++c) q[c] = in.q[c * n + i];
+            if (MODE == kFire) {
+                R u[5], f[5];
+#pragma unroll
+                for (int c = 0; c < 5; ++c) {
+                    u[c] = in.u[c * n + i];
+                    f[c] = in.f[c * n + i];
+                }
+#pragma unroll
+                for (int c = 0; c < 5; ++c) {
+                    const R v = mix == 2 ? R(0) : (mix == 0 ? u[c] : ck * u[c] + cf * f[c]);
+                    q[c] = q[c] + (dt * v + hdt2 * f[c]);
+                    vh[c] = v + hdt * f[c];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 5; ++c) sQ[s][c][hOwn] = q[c];
+            sT[s][hOwn] = t;
+            tOwn = t;
+        }
+        if (hHalo >= 0) {
+            const long long i = colHalo + zoff;
+            R q[5];
+#pragma unroll
+            for (int c = 0; c < 5; ++c) q[c] = in.q[c * n + i];
+            if (MODE == kFire) {
+#pragma unroll
+                for (int c = 0; c < 5; ++c) {
+                    const R u = in.u[c * n + i], f = in.f[c * n + i];
+                    const R v = mix == 2 ? R(0) : (mix == 0 ? u : ck * u + cf * f);
+                    q[c] = q[c] + (dt * v + hdt2 * f);
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 5; ++c) sQ[s][c][hHalo] = q[c];
+            sT[s][hHalo] = A.types[i];
+        }
+    };
+
+    double pF = 0, pP = 0, pV = 0;
+    R vhCur[5], vhNext[5];
+    int tCur = 0, tNext = 0;
+    stage(z0 - 1, 0, vhNext, tNext);
+    stage(z0, 1, vhCur, tCur);
+    for (int z = z0; z < z1; ++z) {
+        const int sm = (z - z0) % 3, sc = (z - z0 + 1) % 3, sp = (z - z0 + 2) % 3;
+        stage(z + 1, sp, vhNext, tNext);
+        __syncthreads();
+        if (active) {
+            const long long i = colOwn + plane * z;
+            R q[5];
+            T.load(sc, hOwn, q);
+            if (tCur >= 0) {
+                R F[5];
+                if (tCur == 0) {
+                    bulk_raw_force<R, TERMS>(A.fc, T, sm, sc, sp, hOwn, q, F);
+                    finish_bulk(A.fc, F);
+                } else {
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) F[c] = R(0);
+                }
+                if (MODE == kFire) {
+                    R u[5];
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) u[c] = vhCur[c] + hdt * F[c];
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) {
+                        out.q[c * n + i] = q[c];
+                        out.u[c * n + i] = u[c];
+                        out.f[c * n + i] = F[c];
+                    }
+                    pF += qt_dot(F, F);
+                    pP += qt_dot(F, u);
+                    pV += qt_dot(u, u);
+                } else {
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) out.f[c * n + i] = F[c];
+                }
+            } else if (MODE == kFire) {
+                // the boundary pass finishes U and F of this site
+#pragma unroll
+                for (int c = 0; c < 5; ++c) {
+                    out.q[c * n + i] = q[c];
+                    out.u[c * n + i] = vhCur[c];
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < 5; ++c) vhCur[c] = vhNext[c];
+        tCur = tNext;
+    }
+    if (MODE == kFire) {
+        block_sum3<kThreads>(pF, pP, pV, sRed);
+        if (tid == 0) {
+            A.partials[3 * (long long)b + 0] = pF;
+            A.partials[3 * (long long)b + 1] = pP;
+            A.partials[3 * (long long)b + 2] = pV;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- boundary pass
+
+template <typename R>
+__device__ __forceinline__ void gload(const R* base, long long n, long long i, R* v) {
+#pragma unroll
+    for (int c = 0; c < 5; ++c) v[c] = base[c * n + i];
+}
+
+// FIRE scalar step of energyMinimizerFIRE::fireStepCPU (energyMinimizerFIRE.cpp:77-145),
+// run by one thread once all partial sums are in.
+__device__ void fire_scalar_step(Ctrl* ctrl, const FireConst& fp, double fnorm, double power, double vnorm,
+                                 double* trace) {
+    const int it = ctrl->iterations + 1;
+    const double forceMax = sqrt(fnorm) / fp.nTotal;
+    const double scaling = fnorm > 0. ? sqrt(vnorm / fnorm) : 0.0;
+    double alpha = ctrl->alpha, dt = ctrl->dt;
+    int nPos = ctrl->nPositive;
+    ctrl->cKeep = 1.0 - alpha;
+    ctrl->cForce = alpha * scaling;
+    ctrl->identity = 0;
+    if (power > 0 && it % 500 != 0) {
+        if (nPos > fp.nMin) {
+            dt = fmin(dt * fp.dtInc, fp.dtMax);
+            alpha = alpha * fp.alphaDec;
+            alpha = fmax(alpha, fp.alphaMin);
+        }
+        nPos += 1;
+        ctrl->reset = 0;
+    } else {
+        nPos = 0;
+        dt = dt * fp.dtDec;
+        dt = fmax(dt, fp.dtMin);
+        alpha = fp.alphaStart;
+        ctrl->reset = 1;
+    }
+    ctrl->alpha = alpha;
+    ctrl->dt = dt;
+    ctrl->nPositive = nPos;
+    ctrl->iterations = it;
+    ctrl->forceMax = forceMax;
+    ctrl->power = power;
+    ctrl->fnorm = fnorm;
+    ctrl->vnorm = vnorm;
+    ctrl->scaling = scaling;
+    if (trace && ctrl->traceLen < ctrl->traceCap) {
+        double* t = trace + 6 * (long long)ctrl->traceLen;
+        t[0] = forceMax;
+        t[1] = power;
+        t[2] = alpha;
+        t[3] = dt;
+        t[4] = scaling;
+        t[5] = nPos;
+        ctrl->traceLen += 1;
+    }
+    ctrl->cur ^= 1;
+    if (isnan(forceMax)) ctrl->nanFlag = 1;
+    ctrl->done = !(it < ctrl->maxIter && forceMax > fp.fc);
+}
+
+template <typename R, int MODE, int TERMS>
+__global__ void __launch_bounds__(kBoundaryThreads) boundary_kernel(BoundaryArgs<R> A) {
+    __shared__ double sRed[kBoundaryThreads / 64][3];
+    __shared__ int sLast;
+    Ctrl* ctrl = A.ctrl;
+    if (MODE == kFire && ctrl->done) return;
+    const int cur = ctrl->cur;
+    const StateSet<R> st = A.set[MODE == kFire ? cur ^ 1 : cur];
+    const R hdt = MODE == kFire ? R(0.5 * ctrl->dt) : R(0);
+    const long long n = A.n;
+    const int lx = A.lx, ly = A.ly, lz = A.lz;
+    double pF = 0, pP = 0, pV = 0;
+
+    const long long k = (long long)blockIdx.x * kBoundaryThreads + threadIdx.x;
+    if (k < A.nsites) {
+        const long long i = A.sites[k];
+        const int self = A.types[i];
+        const int x = int(i % lx), y = int((i / lx) % ly), z = int(i / ((long long)lx * ly));
+        // neighbour sites in the reference order xd, xu, yd, yu, zd, zu
+        long long nbi[6];
+        nbi[0] = wrapi(x - 1, lx) + (long long)lx * (y + (long long)ly * z);
+        nbi[1] = wrapi(x + 1, lx) + (long long)lx * (y + (long long)ly * z);
+        nbi[2] = x + (long long)lx * (wrapi(y - 1, ly) + (long long)ly * z);
+        nbi[3] = x + (long long)lx * (wrapi(y + 1, ly) + (long long)ly * z);
+        nbi[4] = x + (long long)lx * (y + (long long)ly * wrapi(z - 1, lz));
+        nbi[5] = x + (long long)lx * (y + (long long)ly * wrapi(z + 1, lz));
+        R xin[125];
+        int tn[6];
+        gload(st.q, n, i, xin);
+#pragma unroll
+        for (int d = 0; d < 6; ++d) {
+            tn[d] = A.types[nbi[d]];
+            gload(st.q, n, nbi[d], xin + 5 + 5 * d);
+        }
+        // transverse derivatives of each neighbour (along-axis entries are unused)
+#pragma unroll
+        for (int d = 0; d < 6; ++d) {
+            const int axis = d / 2;
+            const int nx = d == 0 ? wrapi(x - 1, lx) : (d == 1 ? wrapi(x + 1, lx) : x);
+            const int ny = d == 2 ? wrapi(y - 1, ly) : (d == 3 ? wrapi(y + 1, ly) : y);
+            const int nz = d == 4 ? wrapi(z - 1, lz) : (d == 5 ? wrapi(z + 1, lz) : z);
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) {
+                R* out = xin + 35 + 15 * d + 5 * ax;
+                if (ax == axis) {
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) out[c] = R(0);
+                    continue;
+                }
+                const long long iu = ax == 0 ? wrapi(nx + 1, lx) + (long long)lx * (ny + (long long)ly * nz)
+                                   : ax == 1 ? nx + (long long)lx * (wrapi(ny + 1, ly) + (long long)ly * nz)
+                                             : nx + (long long)lx * (ny + (long long)ly * wrapi(nz + 1, lz));
+                const long long idn = ax == 0 ? wrapi(nx - 1, lx) + (long long)lx * (ny + (long long)ly * nz)
+                                    : ax == 1 ? nx + (long long)lx * (wrapi(ny - 1, ly) + (long long)ly * nz)
+                                              : nx + (long long)lx * (ny + (long long)ly * wrapi(nz - 1, lz));
+                R qu[5], qd[5];
+                gload(st.q, n, iu, qu);
+                gload(st.q, n, idn, qd);
+                nbr_derivative(tn[d], int(A.types[iu]), int(A.types[idn]), xin + 5 + 5 * d, qu, qd, out);
+            }
+        }
+        const R* q = xin;
+        R F[5];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) F[c] = R(0);
+        phase_force(R(A.fc.a), R(A.fc.b), R(A.fc.c), q, F);
+        R spatial[5];
+        if (self == -1) {
+            // boundaryL1Force: only object-facing directions (lcForces.h:39-63)
+#pragma unroll
+            for (int c = 0; c < 5; ++c) spatial[c] = R(0);
+#pragma unroll
+            for (int d = 0; d < 6; ++d)
+                if (tn[d] > 0) {
+                    const R* opp = xin + 5 + 5 * (d ^ 1);
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) spatial[c] -= (opp[c] - q[c]);
+                }
+#pragma unroll
+            for (int c = 0; c < 5; ++c) spatial[c] *= R(A.fc.L1);
+        } else {
+#pragma unroll
+            for (int c = 0; c < 5; ++c) {
+                R lap = R(6) * q[c];
+#pragma unroll
+                for (int d = 0; d < 6; ++d) lap -= xin[5 + 5 * d + c];
+                spatial[c] = R(A.fc.L1) * lap;
+            }
+        }
+        l1_shape(spatial);
+        if constexpr (TERMS != kOneConstant) {
+            R acc[3][5];
+#pragma unroll
+            for (int t = 0; t < 3; ++t)
+#pragma unroll
+                for (int c = 0; c < 5; ++c) acc[t][c] = R(0);
+            int bcase = 0;
+            if (self == -1) {
+                int xx = 0, yy = 0, zz = 0;
+                if (tn[1] > 0) xx = 1;
+                if (tn[0] > 0) xx = 2;
+                if (tn[3] > 0) yy = 1;
+                if (tn[2] > 0) yy = 2;
+                if (tn[5] > 0) zz = 1;
+                if (tn[4] > 0) zz = 2;
+                bcase = xx + 4 * yy + 16 * zz;
+            }
+            ldg_boundary_lin(bcase, xin, acc);
+#pragma unroll
+            for (int c = 0; c < 5; ++c) spatial[c] += R(A.fc.L2) * acc[0][c];
+#pragma unroll
+            for (int c = 0; c < 5; ++c) spatial[c] += R(A.fc.L3) * acc[1][c];
+#pragma unroll
+            for (int c = 0; c < 5; ++c) spatial[c] += R(A.fc.L4) * acc[2][c];
+            if constexpr (TERMS == kMultiL6) {
+                R o6[5];
+                ldg_boundary_l6(bcase, xin, o6);
+#pragma unroll
+                for (int c = 0; c < 5; ++c) spatial[c] += R(A.fc.L6) * o6[c];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 5; ++c) F[c] -= spatial[c];
+#pragma unroll
+        for (int d = 0; d < 6; ++d)
+            if (tn[d] > 0) add_anchoring_force(A.objects[tn[d] - 1], q, xin + 5 + 5 * d, F);
+        if (A.fc.hasField) {
+#pragma unroll
+            for (int c = 0; c < 5; ++c) F[c] += R(A.fc.field[c]);
+        }
+        metric(F);
+        if (MODE == kFire) {
+            R u[5];
+            gload(st.u, n, i, u);
+#pragma unroll
+            for (int c = 0; c < 5; ++c) {
+                u[c] = u[c] + hdt * F[c];
+                st.u[c * n + i] = u[c];
+                st.f[c * n + i] = F[c];
+            }
+            pF = qt_dot(F, F);
+            pP = qt_dot(F, u);
+            pV = qt_dot(u, u);
+        } else {
+#pragma unroll
+            for (int c = 0; c < 5; ++c) st.f[c * n + i] = F[c];
+        }
+    }
+    if (MODE != kFire) return;
+    block_sum3<kBoundaryThreads>(pF, pP, pV, sRed);
+    if (threadIdx.x == 0) {
+        const long long slot = A.marchBlocks + blockIdx.x;
+        A.partials[3 * slot + 0] = pF;
+        A.partials[3 * slot + 1] = pP;
+        A.partials[3 * slot + 2] = pV;
+        __threadfence();
+        const unsigned prev = __hip_atomic_fetch_add(&ctrl->arrive, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        sLast = prev == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!sLast) return;
+    __threadfence();
+    // fixed-order sum of every partial: march blocks then boundary blocks
+    const long long total = A.marchBlocks + gridDim.x;
+    double s0 = 0, s1 = 0, s2 = 0;
+    for (long long j = threadIdx.x; j < total; j += kBoundaryThreads) {
+        s0 += __hip_atomic_load(&A.partials[3 * j + 0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s1 += __hip_atomic_load(&A.partials[3 * j + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s2 += __hip_atomic_load(&A.partials[3 * j + 2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    block_sum3<kBoundaryThreads>(s0, s1, s2, sRed);
+    if (threadIdx.x == 0) {
+        ctrl->arrive = 0;
+        fire_scalar_step(ctrl, A.fire, s0, s1, s2, A.trace);
+    }
+}
+
+// ---------------------------------------------------------------- other passes
+
+// Materialize the pending FIRE velocity mix into U of the current set.
+template <typename R>
+__global__ void materialize_velocity_kernel(StateSet<R> s0, StateSet<R> s1, long long n, Ctrl* ctrl) {
+    const StateSet<R> st = ctrl->cur ? s1 : s0;
+    const int mix = ctrl->reset ? 2 : (ctrl->identity ? 0 : 1);
+    const R ck = R(ctrl->cKeep), cf = R(ctrl->cForce);
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < 5 * n; i += (long long)gridDim.x * blockDim.x) {
+        const R u = st.u[i];
+        st.u[i] = mix == 2 ? R(0) : (mix == 0 ? u : ck * u + cf * st.f[i]);
+    }
+}
+
+// Energy density per site and its components (landauDeGennesLC.cpp:204-353);
+// per-block partial sums of the five components.
+template <typename R>
+__global__ void __launch_bounds__(kReduceThreads) energy_kernel(const R* q, const int8_t* types,
+                                                                const DevObject* objects, int lx, int ly, int lz,
+                                                                EnergyConst ec, double* partials) {
+    __shared__ double sRed[kReduceThreads / 64][5];
+    const long long n = (long long)lx * ly * lz;
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const int t = types[i];
+        if (t > 0) continue;
+        const int x = int(i % lx), y = int((i / lx) % ly), z = int(i / ((long long)lx * ly));
+        long long nbi[6];
+        nbi[0] = wrapi(x - 1, lx) + (long long)lx * (y + (long long)ly * z);
+        nbi[1] = wrapi(x + 1, lx) + (long long)lx * (y + (long long)ly * z);
+        nbi[2] = x + (long long)lx * (wrapi(y - 1, ly) + (long long)ly * z);
+        nbi[3] = x + (long long)lx * (wrapi(y + 1, ly) + (long long)ly * z);
+        nbi[4] = x + (long long)lx * (y + (long long)ly * wrapi(z - 1, lz));
+        nbi[5] = x + (long long)lx * (y + (long long)ly * wrapi(z + 1, lz));
+        R qc[5], nq[6][5];
+        gload(q, n, i, qc);
+#pragma unroll
+        for (int d = 0; d < 6; ++d) gload(q, n, nbi[d], nq[d]);
+        const R t2 = trQ2(qc);
+        acc[0] += double(R(ec.a) * t2 + R(ec.b) * trQ3(qc) + R(ec.c) * t2 * t2);
+        if (ec.eOn) acc[3] += double(field_energy(ec.E, ec.eps, ec.deps, ec.eps0, qc));
+        if (ec.hOn) acc[4] += double(field_energy(ec.H, ec.chi, ec.dchi, ec.mu0, qc));
+        R dv[3][5];
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax)
+#pragma unroll
+            for (int c = 0; c < 5; ++c) dv[ax][c] = R(0.5) * (nq[2 * ax + 1][c] - nq[2 * ax][c]);
+        if (t < 0) {
+            R anch = 0;
+#pragma unroll
+            for (int d = 0; d < 6; ++d) {
+                const int tn = types[nbi[d]];
+                if (tn <= 0) continue;
+                anch += anchoring_energy(objects[tn - 1], qc, nq[d]);
+                const int ax = d / 2;
+#pragma unroll
+                for (int c = 0; c < 5; ++c)
+                    dv[ax][c] = (d & 1) ? (qc[c] - nq[d ^ 1][c]) : (nq[d ^ 1][c] - qc[c]);
+            }
+            acc[2] += double(anch);
+        }
+        R e[5];
+        distortion_densities(qc, dv[0], dv[1], dv[2], e);
+        double dist = 0;
+#pragma unroll
+        for (int k = 0; k < 5; ++k)
+            if (ec.L[k] != 0) dist += ec.L[k] * double(e[k]);
+        acc[1] += dist;
+    }
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        const double v = wave_sum(acc[k]);
+        if (l == 0) sRed[w][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 5) {
+        double s = 0;
+        for (int i = 0; i < kReduceThreads / 64; ++i) s += sRed[i][threadIdx.x];
+        partials[5 * (long long)blockIdx.x + threadIdx.x] = s;
+    }
+}
+
+// qTensorLatticeModel::computeDefectMeasures (qTensorLatticeModel.cpp:95-133) with
+// eigenvaluesOfQ (qTensorFunctions.h:215-241)
+template <typename R>
+__global__ void defect_kernel(const R* q, const int8_t* types, long long n, int kind, double* out) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        if (types[i] > 0) {
+            out[i] = 0.0;
+            continue;
+        }
+        double v[5];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) v[c] = double(q[c * n + i]);
+        const double w = -v[0] - v[3];
+        const double det = v[0] * (v[3] * w - v[4] * v[4]) - v[1] * (v[1] * w - v[4] * v[2]) +
+                           v[2] * (v[1] * v[4] - v[3] * v[2]);
+        if (kind == 1) {
+            out[i] = det;
+            continue;
+        }
+        if (kind == 2) {
+            const double t = trQ2(v);
+            out[i] = t * t * t - 54.0 * det * det;
+            continue;
+        }
+        const double p1 = v[1] * v[1] + v[2] * v[2] + v[4] * v[4];
+        if (p1 == 0) {
+            out[i] = fmax(fmax(v[0], v[3]), w);
+            continue;
+        }
+        const double p2 = 2 * (v[0] * v[0] + v[3] * v[3] + v[0] * v[3]) + 2 * p1;
+        const double p = sqrt(p2 / 6.0);
+        const double r = 0.5 * det / (p * p * p);
+        const double pi = 3.14159265358979323846;
+        double phi;
+        if (r <= -1)
+            phi = pi / 3.0;
+        else if (r >= 1)
+            phi = 0.0;
+        else
+            phi = acos(r) / 3.0;
+        const double a = 2.0 * p * cos(phi);
+        const double c = 2.0 * p * cos(phi + 2.0 * pi / 3.0);
+        out[i] = fmax(fmax(a, -a - c), c);
+    }
+}
+
+// ---------------------------------------------------------------- host launchers
+
+template <typename R, int MODE, int TERMS>
+static void launch_march_t(const MarchArgs<R>& a, hipStream_t s) {
+    hipLaunchKernelGGL((march_kernel<R, MODE, TERMS>), dim3(a.nblocks), dim3(kThreads), 0, s, a);
+}
+
+template <typename R, int MODE, int TERMS>
+static void launch_boundary_t(const BoundaryArgs<R>& a, int blocks, hipStream_t s) {
+    hipLaunchKernelGGL((boundary_kernel<R, MODE, TERMS>), dim3(blocks), dim3(kBoundaryThreads), 0, s, a);
+}
+
+template <typename R>
+void launch_march(const MarchArgs<R>& a, int mode, int terms, hipStream_t s) {
+    if (mode == kFire) {
+        if (terms == kOneConstant) launch_march_t<R, kFire, kOneConstant>(a, s);
+        else if (terms == kMultiLinear) launch_march_t<R, kFire, kMultiLinear>(a, s);
+        else launch_march_t<R, kFire, kMultiL6>(a, s);
+    } else {
+        if (terms == kOneConstant) launch_march_t<R, kForce, kOneConstant>(a, s);
+        else if (terms == kMultiLinear) launch_march_t<R, kForce, kMultiLinear>(a, s);
+        else launch_march_t<R, kForce, kMultiL6>(a, s);
+    }
+}
+
+template <typename R>
+void launch_boundary(const BoundaryArgs<R>& a, int mode, int terms, hipStream_t s) {
+    long long blocks = (a.nsites + kBoundaryThreads - 1) / kBoundaryThreads;
+    if (blocks < 1) blocks = 1;
+    if (mode == kFire) {
+        if (terms == kOneConstant) launch_boundary_t<R, kFire, kOneConstant>(a, int(blocks), s);
+        else if (terms == kMultiLinear) launch_boundary_t<R, kFire, kMultiLinear>(a, int(blocks), s);
+        else launch_boundary_t<R, kFire, kMultiL6>(a, int(blocks), s);
+    } else {
+        if (a.nsites == 0) return;
+        if (terms == kOneConstant) launch_boundary_t<R, kForce, kOneConstant>(a, int(blocks), s);
+        else if (terms == kMultiLinear) launch_boundary_t<R, kForce, kMultiLinear>(a, int(blocks), s);
+        else launch_boundary_t<R, kForce, kMultiL6>(a, int(blocks), s);
+    }
+}
+
+int boundary_blocks(long long nsites) {
+    long long b = (nsites + kBoundaryThreads - 1) / kBoundaryThreads;
+    return b < 1 ? 1 : int(b);
+}
+
+template <typename R>
+void launch_materialize(StateSet<R> s0, StateSet<R> s1, long long n, Ctrl* ctrl, hipStream_t s) {
+    long long blocks = (5 * n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL((materialize_velocity_kernel<R>), dim3(blocks), dim3(256), 0, s, s0, s1, n, ctrl);
+}
+
+template <typename R>
+int launch_energy(const R* q, const int8_t* types, const DevObject* objects, int lx, int ly, int lz,
+                  const EnergyConst& ec, double* partials, hipStream_t s) {
+    long long n = (long long)lx * ly * lz;
+    long long blocks = (n + kReduceThreads - 1) / kReduceThreads;
+    if (blocks > kMaxEnergyBlocks) blocks = kMaxEnergyBlocks;
+    hipLaunchKernelGGL((energy_kernel<R>), dim3(blocks), dim3(kReduceThreads), 0, s, q, types, objects, lx, ly, lz,
+                       ec, partials);
+    return int(blocks);
+}
+
+template <typename R>
+void launch_defects(const R* q, const int8_t* types, long long n, int kind, double* out, hipStream_t s) {
+    long long blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL((defect_kernel<R>), dim3(blocks), dim3(256), 0, s, q, types, n, kind, out);
+}
+
+#define QMIN_INSTANTIATE(R)                                                                                      \
+    template void launch_march<R>(const MarchArgs<R>&, int, int, hipStream_t);                                   \
+    template void launch_boundary<R>(const BoundaryArgs<R>&, int, int, hipStream_t);                             \
+    template void launch_materialize<R>(StateSet<R>, StateSet<R>, long long, Ctrl*, hipStream_t);                \
+    template int launch_energy<R>(const R*, const int8_t*, const DevObject*, int, int, int, const EnergyConst&,   \
+                                  double*, hipStream_t);                                                        \
+    template void launch_defects<R>(const R*, const int8_t*, long long, int, double*, hipStream_t);
+
+QMIN_INSTANTIATE(double)
+QMIN_INSTANTIATE(float)
+
+}  // namespace qmin

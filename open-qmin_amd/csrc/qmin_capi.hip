@@ -1,0 +1,938 @@
+// Host side of the qmin C ABI (include/qmin.h): lattice state in HBM,
+// objects, the seeded initial state, and the minimizer drivers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <random>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/qmin.h"
+#include "qmin_device.h"
+#include "host_init.h"
+
+namespace qmin {
+template <typename R>
+void launch_march(const MarchArgs<R>& a, int mode, int terms, hipStream_t s);
+template <typename R>
+void launch_boundary(const BoundaryArgs<R>& a, int mode, int terms, hipStream_t s);
+int boundary_blocks(long long nsites);
+template <typename R>
+void launch_materialize(StateSet<R> s0, StateSet<R> s1, long long n, Ctrl* ctrl, hipStream_t s);
+template <typename R>
+int launch_energy(const R* q, const int8_t* types, const DevObject* objects, int lx, int ly, int lz,
+                  const EnergyConst& ec, double* partials, hipStream_t s);
+template <typename R>
+void launch_defects(const R* q, const int8_t* types, long long n, int kind, double* out, hipStream_t s);
+// minimizer passes (minimizers.hip)
+int adam_minimize_impl(qm_adam* m, qm_ldg* f, qm_lattice* lat);
+int nesterov_minimize_impl(qm_nesterov* m, qm_ldg* f, qm_lattice* lat);
+int lolbfgs_minimize_impl(qm_lolbfgs* m, qm_ldg* f, qm_lattice* lat);
+}  // namespace qmin
+
+using namespace qmin;
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string g_error;
+
+static int fail(int code, const std::string& msg) {
+    g_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(QM_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
+// ------------------------------------------------------------------ timing
+namespace {
+struct Timing {
+    bool on = false;
+    int64_t launches[2] = {0, 0};
+    double ms[2] = {0, 0};
+};
+Timing g_timing;
+}  // namespace
+
+// ------------------------------------------------------------------ handles
+struct HostObject {
+    int type;
+    double W, S0;
+};
+
+struct qm_lattice {
+    int lx = 0, ly = 0, lz = 0, precision = QM_FP64;
+    long long n = 0;
+    size_t realSize = 8;
+    hipStream_t stream = nullptr;
+    std::vector<int32_t> types;      // host copy, reference encoding
+    std::vector<HostObject> objects;
+    int8_t* dTypes = nullptr;
+    DevObject* dObjects = nullptr;
+    void* dSet[2] = {nullptr, nullptr};   // each 15 n reals: Q, U, F planes
+    int32_t* dSites = nullptr;            // compacted t < 0 sites
+    long long nBoundary = 0;
+    Ctrl* dCtrl = nullptr;
+    Ctrl hCtrl{};                          // host mirror, valid after syncCtrl
+    double* dPartials = nullptr;
+    size_t partialsCap = 0;
+    double* dTrace = nullptr;
+    int traceCap = 0;
+    bool typesDirty = true;
+    int zchunk = 16;
+};
+
+struct qm_ldg {
+    double A = -1, B = 0, C = 0;
+    double L1 = 2.32, L2 = 0, L3 = 0, L4 = 0, L6 = 0;
+    int multi = 0;
+    int semantics = QM_SEMANTICS_REF_GPU;
+    int eOn = 0, hOn = 0;
+    double E[3] = {0, 0, 0}, eps = 1, eps0 = 1, deps = 0.5;
+    double H[3] = {0, 0, 0}, chi = 1, mu0 = 1, dchi = 0.5;
+};
+
+struct qm_fire {
+    // energyMinimizerFIRE::initializeParameters defaults (energyMinimizerFIRE.cpp:24-43)
+    double dt = 0.01, dtMin = 0.0001, alphaStart = 0.99, alpha = 0.99, dtMax = 0.1, dtInc = 1.05, dtDec = 0.95,
+           alphaDec = 0.9, alphaMin = 0.0, fc = 1e-7;
+    int nMin = 5, maxIter = 1000;
+    int iterations = 0, nPositive = 0;
+    double forceMax = 100.0, power = 0, scaling = 0;
+    int converged = 0, nan = 0;
+    int traceCap = 0;
+    std::vector<double> trace;
+    qm_lattice* bound = nullptr;   // lattice holding this minimizer's device scalars
+};
+
+const char* qm_last_error(void) { return g_error.c_str(); }
+
+int qm_device_count(int* count) {
+    if (!count) return fail(QM_EINVAL, "null pointer");
+    int c = 0;
+    HIP_TRY(hipGetDeviceCount(&c));
+    *count = c;
+    return QM_OK;
+}
+
+// ------------------------------------------------------------------ lattice helpers
+static int64_t site_of(const qm_lattice* L, int x, int y, int z) {
+    return (int64_t)x + (int64_t)L->lx * ((int64_t)y + (int64_t)L->ly * z);
+}
+
+static int wrapc(int v, int m) {
+    // wrap (inc/functions.h:299-307)
+    int a = v;
+    if (v >= m) a = v % m;
+    while (a < 0) a += m;
+    return a;
+}
+
+template <typename R>
+static StateSet<R> state_set(const qm_lattice* L, int s) {
+    R* base = static_cast<R*>(L->dSet[s]);
+    return StateSet<R>{base, base + 5 * L->n, base + 10 * L->n};
+}
+
+static int sync_ctrl_to_host(qm_lattice* L) {
+    HIP_TRY(hipMemcpyAsync(&L->hCtrl, L->dCtrl, sizeof(Ctrl), hipMemcpyDeviceToHost, L->stream));
+    HIP_TRY(hipStreamSynchronize(L->stream));
+    return QM_OK;
+}
+
+static int push_ctrl(qm_lattice* L) {
+    HIP_TRY(hipMemcpyAsync(L->dCtrl, &L->hCtrl, sizeof(Ctrl), hipMemcpyHostToDevice, L->stream));
+    HIP_TRY(hipStreamSynchronize(L->stream));
+    return QM_OK;
+}
+
+// Upload types/objects and rebuild the compacted boundary-site list.
+static int refresh_types(qm_lattice* L) {
+    if (!L->typesDirty) return QM_OK;
+    std::vector<int8_t> t8(L->n);
+    std::vector<int32_t> sites;
+    for (long long i = 0; i < L->n; ++i) {
+        t8[i] = (int8_t)L->types[i];
+        if (L->types[i] < 0) sites.push_back((int32_t)i);
+    }
+    HIP_TRY(hipMemcpy(L->dTypes, t8.data(), L->n, hipMemcpyHostToDevice));
+    std::vector<DevObject> objs(kMaxObjects);
+    for (size_t j = 0; j < L->objects.size(); ++j) objs[j] = {L->objects[j].type, 0, L->objects[j].W, L->objects[j].S0};
+    HIP_TRY(hipMemcpy(L->dObjects, objs.data(), sizeof(DevObject) * kMaxObjects, hipMemcpyHostToDevice));
+    if (L->dSites) HIP_TRY(hipFree(L->dSites));
+    L->dSites = nullptr;
+    L->nBoundary = (long long)sites.size();
+    if (!sites.empty()) {
+        HIP_TRY(hipMalloc(&L->dSites, sites.size() * sizeof(int32_t)));
+        HIP_TRY(hipMemcpy(L->dSites, sites.data(), sites.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
+    L->typesDirty = false;
+    return QM_OK;
+}
+
+static int ensure_partials(qm_lattice* L, size_t count) {
+    if (count <= L->partialsCap) return QM_OK;
+    if (L->dPartials) HIP_TRY(hipFree(L->dPartials));
+    HIP_TRY(hipMalloc(&L->dPartials, count * sizeof(double)));
+    L->partialsCap = count;
+    return QM_OK;
+}
+
+// AoS host field -> SoA device plane set (current set)
+static int upload_field(qm_lattice* L, int which, const double* aos) {
+    const long long n = L->n;
+    if (L->precision == QM_FP64) {
+        std::vector<double> soa(5 * n);
+        for (long long i = 0; i < n; ++i)
+            for (int c = 0; c < 5; ++c) soa[c * n + i] = aos[5 * i + c];
+        double* base = static_cast<double*>(L->dSet[L->hCtrl.cur]) + which * 5 * n;
+        HIP_TRY(hipMemcpy(base, soa.data(), 5 * n * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> soa(5 * n);
+        for (long long i = 0; i < n; ++i)
+            for (int c = 0; c < 5; ++c) soa[c * n + i] = (float)aos[5 * i + c];
+        float* base = static_cast<float*>(L->dSet[L->hCtrl.cur]) + which * 5 * n;
+        HIP_TRY(hipMemcpy(base, soa.data(), 5 * n * sizeof(float), hipMemcpyHostToDevice));
+    }
+    return QM_OK;
+}
+
+static int download_field(qm_lattice* L, int which, double* aos) {
+    const long long n = L->n;
+    HIP_TRY(hipStreamSynchronize(L->stream));
+    if (L->precision == QM_FP64) {
+        std::vector<double> soa(5 * n);
+        const double* base = static_cast<double*>(L->dSet[L->hCtrl.cur]) + which * 5 * n;
+        HIP_TRY(hipMemcpy(soa.data(), base, 5 * n * sizeof(double), hipMemcpyDeviceToHost));
+        for (long long i = 0; i < n; ++i)
+            for (int c = 0; c < 5; ++c) aos[5 * i + c] = soa[c * n + i];
+    } else {
+        std::vector<float> soa(5 * n);
+        const float* base = static_cast<float*>(L->dSet[L->hCtrl.cur]) + which * 5 * n;
+        HIP_TRY(hipMemcpy(soa.data(), base, 5 * n * sizeof(float), hipMemcpyDeviceToHost));
+        for (long long i = 0; i < n; ++i)
+            for (int c = 0; c < 5; ++c) aos[5 * i + c] = soa[c * n + i];
+    }
+    return QM_OK;
+}
+
+// ------------------------------------------------------------------ lattice API
+int qm_lattice_create(int lx, int ly, int lz, int precision, qm_lattice** out) {
+    if (!out) return fail(QM_EINVAL, "null output handle");
+    if (lx < 3 || ly < 3 || lz < 3) return fail(QM_EINVAL, "each lattice edge must be at least 3 sites");
+    if (precision != QM_FP64 && precision != QM_FP32) return fail(QM_EINVAL, "precision must be QM_FP64 or QM_FP32");
+    const long long n = (long long)lx * ly * lz;
+    if (n > (1LL << 31) - 1) return fail(QM_EINVAL, "lattice exceeds 2^31-1 sites");
+    std::unique_ptr<qm_lattice> L(new qm_lattice);
+    L->lx = lx;
+    L->ly = ly;
+    L->lz = lz;
+    L->n = n;
+    L->precision = precision;
+    L->realSize = precision == QM_FP64 ? 8 : 4;
+    L->types.assign(n, 0);
+    HIP_TRY(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking));
+    HIP_TRY(hipMalloc(&L->dTypes, n));
+    HIP_TRY(hipMalloc(&L->dObjects, sizeof(DevObject) * kMaxObjects));
+    for (int s = 0; s < 2; ++s) {
+        HIP_TRY(hipMalloc(&L->dSet[s], 15 * n * L->realSize));
+        HIP_TRY(hipMemset(L->dSet[s], 0, 15 * n * L->realSize));
+    }
+    HIP_TRY(hipMalloc(&L->dCtrl, sizeof(Ctrl)));
+    std::memset(&L->hCtrl, 0, sizeof(Ctrl));
+    L->hCtrl.identity = 1;
+    if (int rc = push_ctrl(L.get())) return rc;
+    if (int rc = refresh_types(L.get())) return rc;
+    *out = L.release();
+    return QM_OK;
+}
+
+int qm_lattice_destroy(qm_lattice* L) {
+    if (!L) return QM_OK;
+    (void)hipStreamSynchronize(L->stream);
+    (void)hipFree(L->dTypes);
+    (void)hipFree(L->dObjects);
+    (void)hipFree(L->dSet[0]);
+    (void)hipFree(L->dSet[1]);
+    if (L->dSites) (void)hipFree(L->dSites);
+    (void)hipFree(L->dCtrl);
+    if (L->dPartials) (void)hipFree(L->dPartials);
+    if (L->dTrace) (void)hipFree(L->dTrace);
+    (void)hipStreamDestroy(L->stream);
+    delete L;
+    return QM_OK;
+}
+
+int qm_lattice_dims(const qm_lattice* L, int* lx, int* ly, int* lz, int* precision) {
+    if (!L) return fail(QM_EINVAL, "null lattice");
+    if (lx) *lx = L->lx;
+    if (ly) *ly = L->ly;
+    if (lz) *lz = L->lz;
+    if (precision) *precision = L->precision;
+    return QM_OK;
+}
+
+int qm_lattice_set_field(qm_lattice* L, int which, const double* aos) {
+    if (!L || !aos || which < 0 || which > 2) return fail(QM_EINVAL, "bad argument");
+    if (which == QM_FIELD_V) {
+        // an explicit velocity replaces any pending FIRE mix
+        L->hCtrl.identity = 1;
+        L->hCtrl.reset = 0;
+        if (int rc = push_ctrl(L)) return rc;
+    }
+    return upload_field(L, which, aos);
+}
+
+static int materialize(qm_lattice* L);
+
+int qm_lattice_get_field(qm_lattice* L, int which, double* aos) {
+    if (!L || !aos || which < 0 || which > 2) return fail(QM_EINVAL, "bad argument");
+    if (which == QM_FIELD_V)
+        if (int rc = materialize(L)) return rc;
+    return download_field(L, which, aos);
+}
+
+// setNematicQTensorRandomly with a reproducible seed (host_init.cpp)
+int qm_lattice_random_init(qm_lattice* L, int seed, double S0) {
+    if (!L) return fail(QM_EINVAL, "null lattice");
+    std::vector<double> q(5 * L->n);
+    if (int rc = download_field(L, QM_FIELD_Q, q.data())) return rc;
+    random_nematic(L->n, L->types.data(), seed, S0, q.data());
+    return upload_field(L, QM_FIELD_Q, q.data());
+}
+
+int qm_lattice_uniform_init(qm_lattice* L, double nx, double ny, double nz, double S0) {
+    if (!L) return fail(QM_EINVAL, "null lattice");
+    std::vector<double> q(5 * L->n);
+    if (int rc = download_field(L, QM_FIELD_Q, q.data())) return rc;
+    double d[5];
+    director_q(nx, ny, nz, S0, d);
+    for (long long i = 0; i < L->n; ++i)
+        if (L->types[i] <= 0) std::memcpy(&q[5 * i], d, sizeof(d));
+    return upload_field(L, QM_FIELD_Q, q.data());
+}
+
+// createMultirankBoundaryObject (multirankSimulationBoundaries.cpp:512-543) then
+// cubicLattice::createBoundaryObject (cubicLattice.cpp:267-318): write Q^B,
+// mark the sites j, mark their non-object neighbours -1.
+int qm_lattice_add_object(qm_lattice* L, const int32_t* xyz, int64_t n, const double* qb, int btype, double W,
+                          double S0, int* obj) {
+    if (!L || (n > 0 && (!xyz || !qb))) return fail(QM_EINVAL, "bad argument");
+    if (btype != QM_HOMEOTROPIC && btype != QM_DEGENERATE_PLANAR) return fail(QM_EINVAL, "unknown boundary type");
+    if ((int)L->objects.size() >= kMaxObjects) return fail(QM_EINVAL, "at most 127 objects per lattice");
+    std::vector<double> q(5 * L->n);
+    if (int rc = download_field(L, QM_FIELD_Q, q.data())) return rc;
+    std::vector<int64_t> sites;
+    sites.reserve(n);
+    for (int64_t k = 0; k < n; ++k) {
+        const int x = wrapc(xyz[3 * k], L->lx), y = wrapc(xyz[3 * k + 1], L->ly), z = wrapc(xyz[3 * k + 2], L->lz);
+        const int64_t i = site_of(L, x, y, z);
+        sites.push_back(i);
+        std::memcpy(&q[5 * i], qb + 5 * k, 5 * sizeof(double));
+    }
+    L->objects.push_back({btype, W, S0});
+    const int j = (int)L->objects.size();
+    for (int64_t i : sites) L->types[i] = j;
+    for (int64_t i : sites) {
+        const int x = int(i % L->lx), y = int((i / L->lx) % L->ly), z = int(i / ((int64_t)L->lx * L->ly));
+        const int64_t nb[6] = {site_of(L, wrapc(x - 1, L->lx), y, z), site_of(L, wrapc(x + 1, L->lx), y, z),
+                               site_of(L, x, wrapc(y - 1, L->ly), z), site_of(L, x, wrapc(y + 1, L->ly), z),
+                               site_of(L, x, y, wrapc(z - 1, L->lz)), site_of(L, x, y, wrapc(z + 1, L->lz))};
+        for (int d = 0; d < 6; ++d)
+            if (L->types[nb[d]] < 1) L->types[nb[d]] = -1;
+    }
+    L->typesDirty = true;
+    if (int rc = refresh_types(L)) return rc;
+    if (obj) *obj = j;
+    return upload_field(L, QM_FIELD_Q, q.data());
+}
+
+int qm_lattice_wall(qm_lattice* L, int xyz, int plane, int btype, double W, double S0, int* obj) {
+    if (!L) return fail(QM_EINVAL, "null lattice");
+    if (xyz < 0 || xyz > 2) return fail(QM_EINVAL, "wall normal must be 0 (x), 1 (y) or 2 (z)");
+    double Qt[5] = {0, 0, 0, 0, 0};
+    if (btype == QM_HOMEOTROPIC) {
+        if (xyz == 0) { Qt[0] = S0; Qt[3] = -0.5 * S0; }
+        else if (xyz == 1) { Qt[0] = -0.5 * S0; Qt[3] = S0; }
+        else { Qt[0] = -0.5 * S0; Qt[3] = -0.5 * S0; }
+    } else {
+        Qt[xyz] = 1.0;
+    }
+    const int s1 = xyz == 0 ? L->ly : L->lx, s2 = xyz == 2 ? L->ly : L->lz;
+    std::vector<int32_t> pts;
+    std::vector<double> qs;
+    pts.reserve(3 * (size_t)s1 * s2);
+    qs.reserve(5 * (size_t)s1 * s2);
+    for (int a = 0; a < s1; ++a)
+        for (int b = 0; b < s2; ++b) {
+            if (xyz == 0) { pts.push_back(plane); pts.push_back(a); pts.push_back(b); }
+            else if (xyz == 1) { pts.push_back(a); pts.push_back(plane); pts.push_back(b); }
+            else { pts.push_back(a); pts.push_back(b); pts.push_back(plane); }
+            qs.insert(qs.end(), Qt, Qt + 5);
+        }
+    return qm_lattice_add_object(L, pts.data(), (int64_t)(pts.size() / 3), qs.data(), btype, W, S0, obj);
+}
+
+int qm_lattice_sphere(qm_lattice* L, double cx, double cy, double cz, double radius, int btype, double W, double S0,
+                      int* obj) {
+    if (!L) return fail(QM_EINVAL, "null lattice");
+    std::vector<int32_t> pts;
+    std::vector<double> qs;
+    sphere_sites(cx, cy, cz, radius, btype, S0, pts, qs);
+    return qm_lattice_add_object(L, pts.data(), (int64_t)(pts.size() / 3), qs.data(), btype, W, S0, obj);
+}
+
+// createBoundaryFromFile (multirankSimulationBoundaries.cpp:41-96): first line the
+// object count; per object a line "a W S0 nSites" (a = 0 oriented, 1 degenerate
+// planar) then nSites lines "x y z Q0 Q1 Q2 Q3 Q4".
+int qm_lattice_boundary_file(qm_lattice* L, const char* path) {
+    if (!L || !path) return fail(QM_EINVAL, "bad argument");
+    std::ifstream in(path);
+    if (!in) return fail(QM_EIO, std::string("cannot open boundary file ") + path);
+    std::string line;
+    if (!std::getline(in, line)) return fail(QM_EIO, "empty boundary file");
+    int nObjects = 0;
+    std::istringstream(line) >> nObjects;
+    for (int o = 0; o < nObjects; ++o) {
+        if (!std::getline(in, line)) return fail(QM_EIO, "boundary file ends before object header");
+        int bt = 0, nEntries = 0;
+        double W = 0, S0 = 0;
+        std::istringstream hs(line);
+        hs >> bt >> W >> S0 >> nEntries;
+        if (!hs) return fail(QM_EIO, "bad object header: " + line);
+        std::vector<int32_t> pts;
+        std::vector<double> qs;
+        int read = 0;
+        while (read < nEntries && std::getline(in, line)) {
+            std::istringstream ls(line);
+            int x, y, z;
+            double q[5];
+            ls >> x >> y >> z >> q[0] >> q[1] >> q[2] >> q[3] >> q[4];
+            pts.push_back(x);
+            pts.push_back(y);
+            pts.push_back(z);
+            qs.insert(qs.end(), q, q + 5);
+            ++read;
+        }
+        if (read != nEntries) return fail(QM_EIO, "boundary file ends inside an object");
+        int rc = qm_lattice_add_object(L, pts.data(), read, qs.data(), bt == 0 ? QM_HOMEOTROPIC : QM_DEGENERATE_PLANAR,
+                                       W, S0, nullptr);
+        if (rc) return rc;
+    }
+    return QM_OK;
+}
+
+int qm_lattice_get_types(const qm_lattice* L, int32_t* types) {
+    if (!L || !types) return fail(QM_EINVAL, "bad argument");
+    std::memcpy(types, L->types.data(), L->n * sizeof(int32_t));
+    return QM_OK;
+}
+
+int qm_lattice_get_neighbors(const qm_lattice* L, int32_t* nb) {
+    if (!L || !nb) return fail(QM_EINVAL, "bad argument");
+    for (int z = 0; z < L->lz; ++z)
+        for (int y = 0; y < L->ly; ++y)
+            for (int x = 0; x < L->lx; ++x) {
+                const int64_t i = site_of(L, x, y, z);
+                int32_t* o = nb + 6 * i;
+                o[0] = (int32_t)site_of(L, wrapc(x - 1, L->lx), y, z);
+                o[1] = (int32_t)site_of(L, wrapc(x + 1, L->lx), y, z);
+                o[2] = (int32_t)site_of(L, x, wrapc(y - 1, L->ly), z);
+                o[3] = (int32_t)site_of(L, x, wrapc(y + 1, L->ly), z);
+                o[4] = (int32_t)site_of(L, x, y, wrapc(z - 1, L->lz));
+                o[5] = (int32_t)site_of(L, x, y, wrapc(z + 1, L->lz));
+            }
+    return QM_OK;
+}
+
+int qm_lattice_n_active(const qm_lattice* L, int64_t* n) {
+    if (!L || !n) return fail(QM_EINVAL, "bad argument");
+    int64_t c = 0;
+    for (long long i = 0; i < L->n; ++i) c += L->types[i] <= 0;
+    *n = c;
+    return QM_OK;
+}
+
+int qm_lattice_n_boundary(const qm_lattice* L, int64_t* n) {
+    if (!L || !n) return fail(QM_EINVAL, "bad argument");
+    int64_t c = 0;
+    for (long long i = 0; i < L->n; ++i) c += L->types[i] < 0;
+    *n = c;
+    return QM_OK;
+}
+
+int qm_lattice_defects(qm_lattice* L, int kind, double* out) {
+    if (!L || !out || kind < 0 || kind > 2) return fail(QM_EINVAL, "bad argument");
+    double* d = nullptr;
+    HIP_TRY(hipMalloc(&d, L->n * sizeof(double)));
+    if (L->precision == QM_FP64)
+        launch_defects<double>(state_set<double>(L, L->hCtrl.cur).q, L->dTypes, L->n, kind, d, L->stream);
+    else
+        launch_defects<float>(state_set<float>(L, L->hCtrl.cur).q, L->dTypes, L->n, kind, d, L->stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, d, L->n * sizeof(double), hipMemcpyDeviceToHost, L->stream));
+    HIP_TRY(hipStreamSynchronize(L->stream));
+    HIP_TRY(hipFree(d));
+    return QM_OK;
+}
+
+// saveState (multirankSimulation.cpp:630-665): "x y z Q0 Q1 Q2 Q3 Q4 type Smax",
+// tab separated, one line per site in index order
+int qm_lattice_save_state(qm_lattice* L, const char* path) {
+    if (!L || !path) return fail(QM_EINVAL, "bad argument");
+    std::vector<double> q(5 * L->n), smax(L->n);
+    if (int rc = qm_lattice_get_field(L, QM_FIELD_Q, q.data())) return rc;
+    if (int rc = qm_lattice_defects(L, 0, smax.data())) return rc;
+    FILE* f = std::fopen(path, "w");
+    if (!f) return fail(QM_EIO, std::string("cannot write ") + path);
+    for (long long i = 0; i < L->n; ++i) {
+        const int x = int(i % L->lx), y = int((i / L->lx) % L->ly), z = int(i / ((long long)L->lx * L->ly));
+        std::fprintf(f, "%i\t%i\t%i\t%f\t%f\t%f\t%f\t%f\t%i\t%f\n", x, y, z, q[5 * i], q[5 * i + 1], q[5 * i + 2],
+                     q[5 * i + 3], q[5 * i + 4], L->types[i], smax[i]);
+    }
+    std::fclose(f);
+    return QM_OK;
+}
+
+// loadState (multirankSimulation.cpp:573-612): reads Q only; types are ignored
+int qm_lattice_load_state(qm_lattice* L, const char* path) {
+    if (!L || !path) return fail(QM_EINVAL, "bad argument");
+    std::ifstream in(path);
+    if (!in) return fail(QM_EIO, std::string("cannot open ") + path);
+    std::vector<double> q(5 * L->n);
+    if (int rc = download_field(L, QM_FIELD_Q, q.data())) return rc;
+    int x, y, z, t;
+    double v[5], s;
+    std::string line;
+    while (std::getline(in, line)) {
+        std::istringstream ls(line);
+        if (!(ls >> x >> y >> z >> v[0] >> v[1] >> v[2] >> v[3] >> v[4])) continue;
+        ls >> t >> s;
+        if (x < 0 || y < 0 || z < 0 || x >= L->lx || y >= L->ly || z >= L->lz)
+            return fail(QM_EIO, "state file site outside the lattice");
+        std::memcpy(&q[5 * site_of(L, x, y, z)], v, sizeof(v));
+    }
+    return upload_field(L, QM_FIELD_Q, q.data());
+}
+
+int qm_lattice_copy_q(qm_lattice* dst, const qm_lattice* src) {
+    if (!dst || !src || dst->n != src->n) return fail(QM_EINVAL, "lattices differ in size");
+    std::vector<double> q(5 * src->n);
+    if (int rc = download_field(const_cast<qm_lattice*>(src), QM_FIELD_Q, q.data())) return rc;
+    return upload_field(dst, QM_FIELD_Q, q.data());
+}
+
+// ------------------------------------------------------------------ force computer
+int qm_ldg_create(qm_ldg** out) {
+    if (!out) return fail(QM_EINVAL, "null output handle");
+    *out = new qm_ldg;
+    return QM_OK;
+}
+int qm_ldg_destroy(qm_ldg* f) {
+    delete f;
+    return QM_OK;
+}
+int qm_ldg_set_phase(qm_ldg* f, double A, double B, double C) {
+    if (!f) return fail(QM_EINVAL, "null force");
+    f->A = A;
+    f->B = B;
+    f->C = C;
+    return QM_OK;
+}
+int qm_ldg_set_elastic(qm_ldg* f, double L1, double L2, double L3, double L4, double L6, int multi) {
+    if (!f) return fail(QM_EINVAL, "null force");
+    if (!multi && (L2 != 0 || L3 != 0 || L4 != 0 || L6 != 0))
+        return fail(QM_EINVAL, "the one-constant path takes L1 only");
+    f->L1 = L1;
+    f->L2 = L2;
+    f->L3 = L3;
+    f->L4 = L4;
+    f->L6 = L6;
+    f->multi = multi;
+    return QM_OK;
+}
+int qm_ldg_set_efield(qm_ldg* f, int on, double Ex, double Ey, double Ez, double eps, double eps0, double deps) {
+    if (!f) return fail(QM_EINVAL, "null force");
+    f->eOn = on;
+    f->E[0] = Ex; f->E[1] = Ey; f->E[2] = Ez;
+    f->eps = eps; f->eps0 = eps0; f->deps = deps;
+    return QM_OK;
+}
+int qm_ldg_set_hfield(qm_ldg* f, int on, double Hx, double Hy, double Hz, double chi, double mu0, double dchi) {
+    if (!f) return fail(QM_EINVAL, "null force");
+    f->hOn = on;
+    f->H[0] = Hx; f->H[1] = Hy; f->H[2] = Hz;
+    f->chi = chi; f->mu0 = mu0; f->dchi = dchi;
+    return QM_OK;
+}
+int qm_ldg_set_semantics(qm_ldg* f, int s) {
+    if (!f || (s != QM_SEMANTICS_REF_CPU && s != QM_SEMANTICS_REF_GPU)) return fail(QM_EINVAL, "bad semantics");
+    f->semantics = s;
+    return QM_OK;
+}
+
+static void field_force(const double* F, double anis, double perm, double* out) {
+    // fieldForce of computeEorHFieldForcesCPU; the site force is F -= fieldForce
+    const double p = anis * perm;
+    out[0] = -0.5 * p * (F[0] * F[0] - F[2] * F[2]);
+    out[1] = -p * F[0] * F[1];
+    out[2] = -p * F[0] * F[2];
+    out[3] = -0.5 * p * (F[1] * F[1] - F[2] * F[2]);
+    out[4] = -p * F[1] * F[2];
+}
+
+static ForceConst force_const(const qm_ldg* f) {
+    ForceConst fc{};
+    fc.a = 0.5 * f->A;
+    fc.b = f->B / 3.0;
+    fc.c = 0.25 * f->C;
+    fc.L1 = f->L1;
+    fc.L2 = f->L2;
+    fc.L3 = f->L3;
+    fc.L4 = f->L4;
+    fc.L6 = f->L6;
+    fc.semantics = f->semantics;
+    double tmp[5];
+    for (int c = 0; c < 5; ++c) fc.field[c] = 0;
+    if (f->eOn) {
+        field_force(f->E, f->deps, f->eps0, tmp);
+        for (int c = 0; c < 5; ++c) fc.field[c] -= tmp[c];
+    }
+    if (f->hOn) {
+        field_force(f->H, f->dchi, f->mu0, tmp);
+        for (int c = 0; c < 5; ++c) fc.field[c] -= tmp[c];
+    }
+    fc.hasField = f->eOn || f->hOn;
+    return fc;
+}
+
+static int terms_of(const qm_ldg* f) {
+    if (!f->multi) return kOneConstant;
+    return f->L6 != 0 ? kMultiL6 : kMultiLinear;
+}
+
+template <typename R>
+static void fill_args(qm_lattice* L, const qm_ldg* f, MarchArgs<R>& ma, BoundaryArgs<R>& ba, const FireConst& fire) {
+    ma.set[0] = state_set<R>(L, 0);
+    ma.set[1] = state_set<R>(L, 1);
+    ma.types = L->dTypes;
+    ma.lx = L->lx;
+    ma.ly = L->ly;
+    ma.lz = L->lz;
+    ma.zchunk = std::min(L->zchunk, L->lz);
+    ma.n = L->n;
+    ma.nbx = (L->lx + kTX - 1) / kTX;
+    ma.nby = (L->ly + kTY - 1) / kTY;
+    const int nbz = (L->lz + ma.zchunk - 1) / ma.zchunk;
+    ma.nblocks = ma.nbx * ma.nby * nbz;
+    ma.fc = force_const(f);
+    ma.ctrl = L->dCtrl;
+    ma.partials = L->dPartials;
+    ba.set[0] = ma.set[0];
+    ba.set[1] = ma.set[1];
+    ba.types = L->dTypes;
+    ba.objects = L->dObjects;
+    ba.sites = L->dSites;
+    ba.nsites = L->nBoundary;
+    ba.lx = L->lx;
+    ba.ly = L->ly;
+    ba.lz = L->lz;
+    ba.n = L->n;
+    ba.marchBlocks = ma.nblocks;
+    ba.fc = ma.fc;
+    ba.fire = fire;
+    ba.ctrl = L->dCtrl;
+    ba.partials = L->dPartials;
+    ba.trace = L->dTrace;
+}
+
+static int march_blocks(const qm_lattice* L) {
+    const int zc = std::min(L->zchunk, L->lz);
+    return ((L->lx + kTX - 1) / kTX) * ((L->ly + kTY - 1) / kTY) * ((L->lz + zc - 1) / zc);
+}
+
+// One pass (force only or one FIRE iteration); kernels only, no host sync.
+template <typename R>
+static int run_pass(qm_lattice* L, const qm_ldg* f, int mode, const FireConst& fire) {
+    MarchArgs<R> ma{};
+    BoundaryArgs<R> ba{};
+    fill_args<R>(L, f, ma, ba, fire);
+    const int terms = terms_of(f);
+    hipEvent_t ev[3];
+    if (g_timing.on && mode == kFire) {
+        for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+        HIP_TRY(hipEventRecord(ev[0], L->stream));
+    }
+    launch_march<R>(ma, mode, terms, L->stream);
+    HIP_TRY(hipGetLastError());
+    if (g_timing.on && mode == kFire) HIP_TRY(hipEventRecord(ev[1], L->stream));
+    launch_boundary<R>(ba, mode, terms, L->stream);
+    HIP_TRY(hipGetLastError());
+    if (g_timing.on && mode == kFire) {
+        HIP_TRY(hipEventRecord(ev[2], L->stream));
+        HIP_TRY(hipEventSynchronize(ev[2]));
+        float a = 0, b = 0;
+        HIP_TRY(hipEventElapsedTime(&a, ev[0], ev[1]));
+        HIP_TRY(hipEventElapsedTime(&b, ev[1], ev[2]));
+        g_timing.launches[0]++;
+        g_timing.launches[1]++;
+        g_timing.ms[0] += a;
+        g_timing.ms[1] += b;
+        for (auto& e : ev) (void)hipEventDestroy(e);
+    }
+    return QM_OK;
+}
+
+static int prepare(qm_lattice* L, const qm_ldg* f) {
+    if (!L || !f) return fail(QM_EINVAL, "null handle");
+    if (int rc = refresh_types(L)) return rc;
+    const size_t need = 3 * (size_t)(march_blocks(L) + boundary_blocks(L->nBoundary));
+    return ensure_partials(L, need);
+}
+
+int qm_ldg_compute_forces(qm_ldg* f, qm_lattice* L) {
+    if (int rc = prepare(L, f)) return rc;
+    FireConst fire{};
+    int rc = L->precision == QM_FP64 ? run_pass<double>(L, f, kForce, fire) : run_pass<float>(L, f, kForce, fire);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(L->stream));
+    return QM_OK;
+}
+
+int qm_ldg_compute_energy(qm_ldg* f, qm_lattice* L, double* total, double* comps) {
+    if (int rc = prepare(L, f)) return rc;
+    EnergyConst ec{};
+    ec.a = 0.5 * f->A;
+    ec.b = f->B / 3.0;
+    ec.c = 0.25 * f->C;
+    ec.L[0] = f->L1;
+    ec.L[1] = f->multi ? f->L2 : 0;
+    ec.L[2] = f->multi ? f->L3 : 0;
+    ec.L[3] = f->multi ? f->L4 : 0;
+    ec.L[4] = f->multi ? f->L6 : 0;
+    ec.eOn = f->eOn;
+    ec.hOn = f->hOn;
+    for (int i = 0; i < 3; ++i) {
+        ec.E[i] = f->E[i];
+        ec.H[i] = f->H[i];
+    }
+    ec.eps = f->eps;
+    ec.eps0 = f->eps0;
+    ec.deps = f->deps;
+    ec.chi = f->chi;
+    ec.mu0 = f->mu0;
+    ec.dchi = f->dchi;
+    if (int rc = ensure_partials(L, 5 * (size_t)kMaxEnergyBlocks)) return rc;
+    int blocks;
+    if (L->precision == QM_FP64)
+        blocks = launch_energy<double>(state_set<double>(L, L->hCtrl.cur).q, L->dTypes, L->dObjects, L->lx, L->ly,
+                                       L->lz, ec, L->dPartials, L->stream);
+    else
+        blocks = launch_energy<float>(state_set<float>(L, L->hCtrl.cur).q, L->dTypes, L->dObjects, L->lx, L->ly,
+                                      L->lz, ec, L->dPartials, L->stream);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> part(5 * (size_t)blocks);
+    HIP_TRY(hipMemcpyAsync(part.data(), L->dPartials, part.size() * sizeof(double), hipMemcpyDeviceToHost, L->stream));
+    HIP_TRY(hipStreamSynchronize(L->stream));
+    double c5[5] = {0, 0, 0, 0, 0};
+    for (int b = 0; b < blocks; ++b)
+        for (int k = 0; k < 5; ++k) c5[k] += part[5 * (size_t)b + k];
+    if (comps)
+        for (int k = 0; k < 5; ++k) comps[k] = c5[k];
+    if (total) *total = c5[0] + c5[1] + c5[2] + c5[3] + c5[4];
+    return QM_OK;
+}
+
+// ------------------------------------------------------------------ FIRE
+static int materialize(qm_lattice* L) {
+    if (int rc = sync_ctrl_to_host(L)) return rc;
+    if (L->hCtrl.identity) return QM_OK;
+    if (L->precision == QM_FP64)
+        launch_materialize<double>(state_set<double>(L, 0), state_set<double>(L, 1), L->n, L->dCtrl, L->stream);
+    else
+        launch_materialize<float>(state_set<float>(L, 0), state_set<float>(L, 1), L->n, L->dCtrl, L->stream);
+    HIP_TRY(hipGetLastError());
+    L->hCtrl.identity = 1;
+    L->hCtrl.reset = 0;
+    HIP_TRY(hipMemcpyAsync(&L->dCtrl->identity, &L->hCtrl.identity, sizeof(int), hipMemcpyHostToDevice, L->stream));
+    HIP_TRY(hipMemcpyAsync(&L->dCtrl->reset, &L->hCtrl.reset, sizeof(int), hipMemcpyHostToDevice, L->stream));
+    HIP_TRY(hipStreamSynchronize(L->stream));
+    return QM_OK;
+}
+
+int qm_fire_create(qm_fire** out) {
+    if (!out) return fail(QM_EINVAL, "null output handle");
+    *out = new qm_fire;
+    return QM_OK;
+}
+int qm_fire_destroy(qm_fire* m) {
+    delete m;
+    return QM_OK;
+}
+
+int qm_fire_set_parameters(qm_fire* m, double dt, double alphaStart, double dtMax, double dtInc, double dtDec,
+                           double alphaDec, int nMin, double fc, double alphaMin) {
+    if (!m) return fail(QM_EINVAL, "null minimizer");
+    m->dt = dt;
+    m->dtMin = dt * .01;
+    m->alphaStart = alphaStart;
+    m->alpha = alphaStart;
+    m->dtMax = dtMax;
+    m->dtInc = dtInc;
+    m->dtDec = dtDec;
+    m->alphaDec = alphaDec;
+    m->nMin = nMin;
+    m->fc = fc;
+    m->alphaMin = alphaMin;
+    return QM_OK;
+}
+
+int qm_fire_set_max_iterations(qm_fire* m, int it) {
+    if (!m) return fail(QM_EINVAL, "null minimizer");
+    m->maxIter = it;
+    return QM_OK;
+}
+
+int qm_fire_set_trace(qm_fire* m, int capacity) {
+    if (!m || capacity < 0) return fail(QM_EINVAL, "bad argument");
+    m->traceCap = capacity;
+    m->trace.clear();
+    return QM_OK;
+}
+
+int qm_fire_get_trace(qm_fire* m, double* rows, int cap, int* n) {
+    if (!m || !n) return fail(QM_EINVAL, "bad argument");
+    const int k = std::min<int>(cap, (int)(m->trace.size() / 6));
+    if (rows && k > 0) std::memcpy(rows, m->trace.data(), 6 * (size_t)k * sizeof(double));
+    *n = k;
+    return QM_OK;
+}
+
+int qm_fire_get_state(const qm_fire* m, qm_fire_state* s) {
+    if (!m || !s) return fail(QM_EINVAL, "bad argument");
+    s->iterations = m->iterations;
+    s->nPositive = m->nPositive;
+    s->converged = m->converged;
+    s->nanDetected = m->nan;
+    s->forceMax = m->forceMax;
+    s->power = m->power;
+    s->alpha = m->alpha;
+    s->deltaT = m->dt;
+    s->scaling = m->scaling;
+    return QM_OK;
+}
+
+static FireConst fire_const(const qm_fire* m, const qm_lattice* L) {
+    FireConst fp{};
+    fp.alphaStart = m->alphaStart;
+    fp.dtMax = m->dtMax;
+    fp.dtInc = m->dtInc;
+    fp.dtDec = m->dtDec;
+    fp.alphaDec = m->alphaDec;
+    fp.alphaMin = m->alphaMin;
+    fp.dtMin = m->dtMin;
+    fp.fc = m->fc;
+    int64_t na = 0;
+    qm_lattice_n_active(L, &na);
+    fp.nTotal = (double)na;
+    fp.nMin = m->nMin;
+    return fp;
+}
+
+// Runs iterations on the device until the control block reports done, syncing
+// with the host once per batch.
+static int fire_drive(qm_fire* m, qm_ldg* f, qm_lattice* L, int maxIter, bool tolerance) {
+    if (int rc = prepare(L, f)) return rc;
+    if (m->traceCap > 0 && L->traceCap < m->traceCap) {
+        if (L->dTrace) HIP_TRY(hipFree(L->dTrace));
+        HIP_TRY(hipMalloc(&L->dTrace, 6 * (size_t)m->traceCap * sizeof(double)));
+        L->traceCap = m->traceCap;
+    }
+    if (int rc = sync_ctrl_to_host(L)) return rc;
+    Ctrl& c = L->hCtrl;
+    c.done = 0;
+    c.iterations = m->iterations;
+    c.maxIter = maxIter;
+    c.nPositive = m->nPositive;
+    c.dt = m->dt;
+    c.alpha = m->alpha;
+    c.arrive = 0;
+    c.nanFlag = 0;
+    c.traceLen = 0;
+    c.traceCap = m->traceCap;
+    c.forceMax = m->forceMax;
+    if (int rc = push_ctrl(L)) return rc;
+    FireConst fp = fire_const(m, L);
+    if (!tolerance) fp.fc = -1.0;  // never converged: exactly maxIter - iterations steps
+    // sim->computeForces() at the start of minimize (energyMinimizerFIRE.cpp:157)
+    int rc = L->precision == QM_FP64 ? run_pass<double>(L, f, kForce, fp) : run_pass<float>(L, f, kForce, fp);
+    if (rc) return rc;
+    const int batch = L->n >= (1 << 22) ? 16 : 64;
+    while (true) {
+        int todo = std::min(batch, maxIter - L->hCtrl.iterations);
+        if (todo <= 0) todo = 1;  // at least one iteration per minimize call
+        for (int k = 0; k < todo; ++k) {
+            rc = L->precision == QM_FP64 ? run_pass<double>(L, f, kFire, fp) : run_pass<float>(L, f, kFire, fp);
+            if (rc) return rc;
+        }
+        if ((rc = sync_ctrl_to_host(L))) return rc;
+        if (L->hCtrl.done) break;
+    }
+    const Ctrl& h = L->hCtrl;
+    m->iterations = h.iterations;
+    m->nPositive = h.nPositive;
+    m->dt = h.dt;
+    m->alpha = h.alpha;
+    m->forceMax = h.forceMax;
+    m->power = h.power;
+    m->scaling = h.scaling;
+    m->nan = h.nanFlag;
+    m->converged = !h.nanFlag && h.forceMax <= m->fc;
+    if (m->traceCap > 0 && h.traceLen > 0) {
+        const size_t old = m->trace.size();
+        m->trace.resize(old + 6 * (size_t)h.traceLen);
+        HIP_TRY(hipMemcpy(m->trace.data() + old, L->dTrace, 6 * (size_t)h.traceLen * sizeof(double),
+                          hipMemcpyDeviceToHost));
+    }
+    if (h.nanFlag) return fail(QM_ENAN, "FIRE produced a NaN force norm");
+    return QM_OK;
+}
+
+int qm_fire_minimize(qm_fire* m, qm_ldg* f, qm_lattice* L) {
+    if (!m) return fail(QM_EINVAL, "null minimizer");
+    return fire_drive(m, f, L, m->maxIter, true);
+}
+
+int qm_fire_run_iterations(qm_fire* m, qm_ldg* f, qm_lattice* L, int n) {
+    if (!m || n < 1) return fail(QM_EINVAL, "bad argument");
+    return fire_drive(m, f, L, m->iterations + n, false);
+}
+
+// ------------------------------------------------------------------ timing
+int qm_timing_enable(int on) {
+    g_timing.on = on != 0;
+    return QM_OK;
+}
+int qm_timing_reset(void) {
+    g_timing.launches[0] = g_timing.launches[1] = 0;
+    g_timing.ms[0] = g_timing.ms[1] = 0;
+    return QM_OK;
+}
+int qm_timing_get(int k, int64_t* launches, double* ms) {
+    if (k < 0 || k > 1) return fail(QM_EINVAL, "kernel index is 0 or 1");
+    if (launches) *launches = g_timing.launches[k];
+    if (ms) *ms = g_timing.ms[k];
+    return QM_OK;
+}
+
+// ------------------------------------------------------------------ Adam / Nesterov / LoLBFGS handles
+// (passes in minimizers.hip)

@@ -1,0 +1,109 @@
+// Shared host/device definitions of the qmin HIP library (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace qmin {
+
+// Lattice tile of the fused pass: 32 x 8 sites per work-group (4 wave64), with a
+// one-site halo in x and y; the work-group marches along z through a chunk.
+constexpr int kTX = 32;
+constexpr int kTY = 8;
+constexpr int kThreads = kTX * kTY;
+constexpr int kHX = kTX + 2;
+constexpr int kHY = kTY + 2;
+constexpr int kHP = kHX * kHY;          // sites of one halo-padded plane
+constexpr int kHalo = kHP - kThreads;   // perimeter sites of one plane
+constexpr int kMaxObjects = 127;        // types are int8 on the device
+constexpr int kBoundaryThreads = 128;
+constexpr int kReduceThreads = 256;
+constexpr int kMaxEnergyBlocks = 4096;
+
+enum Mode { kForce = 0, kFire = 1 };
+enum Terms { kOneConstant = 0, kMultiLinear = 1, kMultiL6 = 2 };
+
+struct DevObject {
+    int type;  // 0 homeotropic, 1 degenerate planar
+    int pad;
+    double P1, P2;
+};
+
+struct ForceConst {
+    double a, b, c;        // A/2, B/3, C/4 of the phase energy
+    double L1, L2, L3, L4, L6;
+    double field[5];       // uniform E and H field force, added to the raw force
+    int hasField;
+    int semantics;         // 0 reference CPU two-pass, 1 reference GPU single pass
+};
+
+// Device-resident control block of one lattice: which state set is current and
+// the FIRE scalars, advanced by the last work-group of each iteration.
+struct Ctrl {
+    int cur;
+    int done;
+    int iterations;
+    int maxIter;
+    int nPositive;
+    int reset;      // pending velocity reset (v = 0) from the last FIRE step
+    int identity;   // no pending velocity mix (v = u)
+    int nanFlag;
+    unsigned int arrive;
+    int traceLen;
+    int traceCap;
+    int pad;
+    double dt, alpha;
+    double cKeep, cForce;   // pending mix v = cKeep u + cForce F
+    double forceMax, power, fnorm, vnorm, scaling;
+};
+
+struct FireConst {
+    double alphaStart, dtMax, dtInc, dtDec, alphaDec, alphaMin, dtMin, fc, nTotal;
+    int nMin;
+    int pad;
+};
+
+struct EnergyConst {
+    double a, b, c;
+    double L[5];  // L1, L2, L3, L4, L6
+    int eOn, hOn;
+    double E[3], eps, eps0, deps;
+    double H[3], chi, mu0, dchi;
+};
+
+template <typename R>
+struct StateSet {
+    R* q;  // 5 planes of n
+    R* u;
+    R* f;
+};
+
+template <typename R>
+struct MarchArgs {
+    StateSet<R> set[2];
+    const int8_t* types;
+    int lx, ly, lz, zchunk;
+    long long n;
+    int nbx, nby, nblocks;
+    ForceConst fc;
+    Ctrl* ctrl;
+    double* partials;   // [nblocks + boundary blocks][3]
+};
+
+template <typename R>
+struct BoundaryArgs {
+    StateSet<R> set[2];
+    const int8_t* types;
+    const DevObject* objects;
+    const int32_t* sites;   // compacted list of t < 0 sites
+    long long nsites;
+    int lx, ly, lz;
+    long long n;
+    int marchBlocks;        // partial slots used by the fused pass
+    ForceConst fc;
+    FireConst fire;
+    Ctrl* ctrl;
+    double* partials;
+    double* trace;          // [traceCap][6]
+};
+
+}  // namespace qmin

@@ -1,0 +1,415 @@
+"""Python binding of libqmin.so (the HIP library) over its C ABI, include/qmin.h.
+
+The classes mirror the reference's objects: a lattice model
+(multirankQTensorLatticeModel on one rank), the landauDeGennesLC force computer
+and the energyMinimizerFIRE / Adam / NesterovAG / LoLBFGS minimizers.  Every
+call runs on the GPU; there is no CPU fallback.  A missing library raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(HERE), "lib", "libqmin.so")
+
+QM_FP64, QM_FP32 = 0, 1
+HOMEOTROPIC, DEGENERATE_PLANAR = 0, 1
+SEMANTICS_REF_CPU, SEMANTICS_REF_GPU = 0, 1
+FIELD_Q, FIELD_V, FIELD_F = 0, 1, 2
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int32)
+_L = ctypes.POINTER(ctypes.c_int64)
+_VP = ctypes.c_void_p
+_i, _d = ctypes.c_int, ctypes.c_double
+
+
+class QminError(RuntimeError):
+    pass
+
+
+class FireState(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("nPositive", ctypes.c_int), ("converged", ctypes.c_int),
+                ("nanDetected", ctypes.c_int), ("forceMax", ctypes.c_double), ("power", ctypes.c_double),
+                ("alpha", ctypes.c_double), ("deltaT", ctypes.c_double), ("scaling", ctypes.c_double)]
+
+
+# name -> argtypes (all return int status)
+_API = {
+    "qm_device_count": [ctypes.POINTER(ctypes.c_int)],
+    "qm_lattice_create": [_i, _i, _i, _i, ctypes.POINTER(_VP)],
+    "qm_lattice_destroy": [_VP],
+    "qm_lattice_dims": [_VP] + [ctypes.POINTER(ctypes.c_int)] * 4,
+    "qm_lattice_set_field": [_VP, _i, _D],
+    "qm_lattice_get_field": [_VP, _i, _D],
+    "qm_lattice_random_init": [_VP, _i, _d],
+    "qm_lattice_uniform_init": [_VP, _d, _d, _d, _d],
+    "qm_lattice_add_object": [_VP, _I, ctypes.c_int64, _D, _i, _d, _d, ctypes.POINTER(ctypes.c_int)],
+    "qm_lattice_wall": [_VP, _i, _i, _i, _d, _d, ctypes.POINTER(ctypes.c_int)],
+    "qm_lattice_sphere": [_VP, _d, _d, _d, _d, _i, _d, _d, ctypes.POINTER(ctypes.c_int)],
+    "qm_lattice_boundary_file": [_VP, ctypes.c_char_p],
+    "qm_lattice_get_types": [_VP, _I],
+    "qm_lattice_get_neighbors": [_VP, _I],
+    "qm_lattice_n_active": [_VP, _L],
+    "qm_lattice_n_boundary": [_VP, _L],
+    "qm_lattice_defects": [_VP, _i, _D],
+    "qm_lattice_save_state": [_VP, ctypes.c_char_p],
+    "qm_lattice_load_state": [_VP, ctypes.c_char_p],
+    "qm_lattice_copy_q": [_VP, _VP],
+    "qm_ldg_create": [ctypes.POINTER(_VP)],
+    "qm_ldg_destroy": [_VP],
+    "qm_ldg_set_phase": [_VP, _d, _d, _d],
+    "qm_ldg_set_elastic": [_VP, _d, _d, _d, _d, _d, _i],
+    "qm_ldg_set_efield": [_VP, _i] + [_d] * 6,
+    "qm_ldg_set_hfield": [_VP, _i] + [_d] * 6,
+    "qm_ldg_set_semantics": [_VP, _i],
+    "qm_ldg_compute_forces": [_VP, _VP],
+    "qm_ldg_compute_energy": [_VP, _VP, _D, _D],
+    "qm_fire_create": [ctypes.POINTER(_VP)],
+    "qm_fire_destroy": [_VP],
+    "qm_fire_set_parameters": [_VP, _d, _d, _d, _d, _d, _d, _i, _d, _d],
+    "qm_fire_set_max_iterations": [_VP, _i],
+    "qm_fire_minimize": [_VP, _VP, _VP],
+    "qm_fire_run_iterations": [_VP, _VP, _VP, _i],
+    "qm_fire_get_state": [_VP, ctypes.POINTER(FireState)],
+    "qm_fire_set_trace": [_VP, _i],
+    "qm_fire_get_trace": [_VP, _D, _i, ctypes.POINTER(ctypes.c_int)],
+    "qm_adam_create": [ctypes.POINTER(_VP)],
+    "qm_adam_destroy": [_VP],
+    "qm_adam_set_parameters": [_VP, _d, _d, _d, _d, _d],
+    "qm_adam_set_max_iterations": [_VP, _i],
+    "qm_adam_minimize": [_VP, _VP, _VP],
+    "qm_adam_get_state": [_VP, ctypes.POINTER(ctypes.c_int), _D],
+    "qm_nesterov_create": [ctypes.POINTER(_VP)],
+    "qm_nesterov_destroy": [_VP],
+    "qm_nesterov_set_parameters": [_VP, _d, _d, _d, _i],
+    "qm_nesterov_set_max_iterations": [_VP, _i],
+    "qm_nesterov_minimize": [_VP, _VP, _VP],
+    "qm_nesterov_get_state": [_VP, ctypes.POINTER(ctypes.c_int), _D],
+    "qm_lolbfgs_create": [ctypes.POINTER(_VP)],
+    "qm_lolbfgs_destroy": [_VP],
+    "qm_lolbfgs_set_parameters": [_VP, _i, _d, _d, _d, _d, _i],
+    "qm_lolbfgs_set_max_iterations": [_VP, _i],
+    "qm_lolbfgs_minimize": [_VP, _VP, _VP],
+    "qm_lolbfgs_get_state": [_VP, ctypes.POINTER(ctypes.c_int), _D],
+    "qm_timing_enable": [_i],
+    "qm_timing_reset": [],
+    "qm_timing_get": [_i, _L, _D],
+}
+
+_lib = None
+
+
+def library(path=LIB_PATH):
+    """Load libqmin.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise QminError("libqmin.so not built: run python open-qmin_amd/build.py")
+        lib = ctypes.CDLL(path)
+        lib.qm_last_error.restype = ctypes.c_char_p
+        lib.qm_last_error.argtypes = []
+        for name, args in _API.items():
+            f = getattr(lib, name)
+            f.argtypes = args
+            f.restype = ctypes.c_int
+        _lib = lib
+    return _lib
+
+
+def exported_symbols():
+    return ["qm_last_error"] + list(_API)
+
+
+def _check(rc):
+    if rc != 0:
+        raise QminError("qmin error %d: %s" % (rc, _lib.qm_last_error().decode()))
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D)
+
+
+class Lattice:
+    """One periodic lattice resident on the GPU (types, objects, Q, v, F)."""
+
+    def __init__(self, lx, ly=None, lz=None, precision=QM_FP64):
+        ly = lx if ly is None else ly
+        lz = lx if lz is None else lz
+        self.lib = library()
+        self.shape = (lx, ly, lz)
+        self.n = lx * ly * lz
+        self.precision = precision
+        h = _VP()
+        _check(self.lib.qm_lattice_create(lx, ly, lz, precision, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.qm_lattice_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_field(self, which, arr):
+        a = np.ascontiguousarray(arr, np.float64).reshape(self.n, 5)
+        _check(self.lib.qm_lattice_set_field(self.h, which, _dp(a)))
+
+    def get_field(self, which):
+        a = np.zeros((self.n, 5))
+        _check(self.lib.qm_lattice_get_field(self.h, which, _dp(a)))
+        return a
+
+    def q(self):
+        return self.get_field(FIELD_Q)
+
+    def v(self):
+        return self.get_field(FIELD_V)
+
+    def f(self):
+        return self.get_field(FIELD_F)
+
+    def set_q(self, q):
+        self.set_field(FIELD_Q, q)
+
+    def random_init(self, seed, S0):
+        _check(self.lib.qm_lattice_random_init(self.h, seed, S0))
+
+    def uniform_init(self, director, S0):
+        _check(self.lib.qm_lattice_uniform_init(self.h, director[0], director[1], director[2], S0))
+
+    def add_object(self, xyz, qb, btype, W, S0):
+        xyz = np.ascontiguousarray(xyz, np.int32)
+        qb = np.ascontiguousarray(qb, np.float64)
+        j = ctypes.c_int()
+        _check(self.lib.qm_lattice_add_object(self.h, xyz.ctypes.data_as(_I), len(xyz), _dp(qb), btype, W, S0,
+                                              ctypes.byref(j)))
+        return j.value
+
+    def wall(self, xyz, plane, btype, W, S0):
+        j = ctypes.c_int()
+        _check(self.lib.qm_lattice_wall(self.h, xyz, plane, btype, W, S0, ctypes.byref(j)))
+        return j.value
+
+    def sphere(self, center, radius, btype, W, S0):
+        j = ctypes.c_int()
+        _check(self.lib.qm_lattice_sphere(self.h, center[0], center[1], center[2], radius, btype, W, S0,
+                                          ctypes.byref(j)))
+        return j.value
+
+    def boundary_file(self, path):
+        _check(self.lib.qm_lattice_boundary_file(self.h, path.encode()))
+
+    def types(self):
+        t = np.zeros(self.n, np.int32)
+        _check(self.lib.qm_lattice_get_types(self.h, t.ctypes.data_as(_I)))
+        return t
+
+    def neighbors(self):
+        nb = np.zeros((self.n, 6), np.int32)
+        _check(self.lib.qm_lattice_get_neighbors(self.h, nb.ctypes.data_as(_I)))
+        return nb
+
+    def n_active(self):
+        v = ctypes.c_int64()
+        _check(self.lib.qm_lattice_n_active(self.h, ctypes.byref(v)))
+        return v.value
+
+    def n_boundary(self):
+        v = ctypes.c_int64()
+        _check(self.lib.qm_lattice_n_boundary(self.h, ctypes.byref(v)))
+        return v.value
+
+    def defects(self, kind=0):
+        out = np.zeros(self.n)
+        _check(self.lib.qm_lattice_defects(self.h, kind, _dp(out)))
+        return out
+
+    def save_state(self, path):
+        _check(self.lib.qm_lattice_save_state(self.h, path.encode()))
+
+    def load_state(self, path):
+        _check(self.lib.qm_lattice_load_state(self.h, path.encode()))
+
+
+class LandauDeGennesLC:
+    """landauDeGennesLC: phase, L1..L6 distortion, anchoring and field terms."""
+
+    def __init__(self, A=-1.0, B=0.0, C=0.0, L1=2.32, L2=0.0, L3=0.0, L4=0.0, L6=0.0, multi=None,
+                 semantics=SEMANTICS_REF_GPU):
+        self.lib = library()
+        h = _VP()
+        _check(self.lib.qm_ldg_create(ctypes.byref(h)))
+        self.h = h
+        self.set_phase(A, B, C)
+        self.set_elastic(L1, L2, L3, L4, L6, multi)
+        self.set_semantics(semantics)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.qm_ldg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_phase(self, A, B, C):
+        _check(self.lib.qm_ldg_set_phase(self.h, A, B, C))
+
+    def set_elastic(self, L1, L2=0.0, L3=0.0, L4=0.0, L6=0.0, multi=None):
+        if multi is None:
+            multi = int(L2 != 0 or L3 != 0 or L4 != 0 or L6 != 0)
+        _check(self.lib.qm_ldg_set_elastic(self.h, L1, L2, L3, L4, L6, int(multi)))
+
+    def set_efield(self, E, eps=1.0, eps0=1.0, deps=0.5, on=True):
+        _check(self.lib.qm_ldg_set_efield(self.h, int(on), E[0], E[1], E[2], eps, eps0, deps))
+
+    def set_hfield(self, H, chi=1.0, mu0=1.0, dchi=0.5, on=True):
+        _check(self.lib.qm_ldg_set_hfield(self.h, int(on), H[0], H[1], H[2], chi, mu0, dchi))
+
+    def set_semantics(self, s):
+        _check(self.lib.qm_ldg_set_semantics(self.h, s))
+
+    def compute_forces(self, lattice):
+        _check(self.lib.qm_ldg_compute_forces(self.h, lattice.h))
+        return lattice.f()
+
+    def compute_energy(self, lattice):
+        tot = ctypes.c_double()
+        comps = np.zeros(5)
+        _check(self.lib.qm_ldg_compute_energy(self.h, lattice.h, ctypes.byref(tot), _dp(comps)))
+        return tot.value, comps
+
+
+class FIRE:
+    """energyMinimizerFIRE on the device."""
+
+    def __init__(self, dt=5e-4, alpha_start=0.99, dt_max=None, dt_inc=1.1, dt_dec=0.95, alpha_dec=0.9, n_min=4,
+                 fc=1e-12, alpha_min=0.0, max_iter=100):
+        self.lib = library()
+        h = _VP()
+        _check(self.lib.qm_fire_create(ctypes.byref(h)))
+        self.h = h
+        dt_max = 100 * dt if dt_max is None else dt_max
+        _check(self.lib.qm_fire_set_parameters(self.h, dt, alpha_start, dt_max, dt_inc, dt_dec, alpha_dec, n_min, fc,
+                                               alpha_min))
+        self.set_max_iterations(max_iter)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.qm_fire_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_max_iterations(self, n):
+        _check(self.lib.qm_fire_set_max_iterations(self.h, n))
+
+    def set_trace(self, capacity):
+        _check(self.lib.qm_fire_set_trace(self.h, capacity))
+
+    def trace(self, cap=1 << 20):
+        st = self.state()
+        cap = min(cap, max(st.iterations, 1))
+        rows = np.zeros((cap, 6))
+        n = ctypes.c_int()
+        _check(self.lib.qm_fire_get_trace(self.h, _dp(rows), cap, ctypes.byref(n)))
+        return rows[:n.value]
+
+    def minimize(self, force, lattice):
+        _check(self.lib.qm_fire_minimize(self.h, force.h, lattice.h))
+        return self.state()
+
+    def run_iterations(self, force, lattice, n):
+        _check(self.lib.qm_fire_run_iterations(self.h, force.h, lattice.h, n))
+        return self.state()
+
+    def state(self):
+        s = FireState()
+        _check(self.lib.qm_fire_get_state(self.h, ctypes.byref(s)))
+        return s
+
+
+class _SimpleMinimizer:
+    _prefix = None
+
+    def __init__(self):
+        self.lib = library()
+        h = _VP()
+        _check(getattr(self.lib, self._prefix + "_create")(ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            getattr(self.lib, self._prefix + "_destroy")(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_max_iterations(self, n):
+        _check(getattr(self.lib, self._prefix + "_set_max_iterations")(self.h, n))
+
+    def minimize(self, force, lattice):
+        _check(getattr(self.lib, self._prefix + "_minimize")(self.h, force.h, lattice.h))
+        return self.state()
+
+    def state(self):
+        it = ctypes.c_int()
+        fm = ctypes.c_double()
+        _check(getattr(self.lib, self._prefix + "_get_state")(self.h, ctypes.byref(it), ctypes.byref(fm)))
+        return it.value, fm.value
+
+
+class Adam(_SimpleMinimizer):
+    """energyMinimizerAdam on the device."""
+    _prefix = "qm_adam"
+
+    def __init__(self, beta1=0.9, beta2=0.99, eps=1e-8, dt=1e-4, fc=1e-12, max_iter=1000):
+        super().__init__()
+        _check(self.lib.qm_adam_set_parameters(self.h, beta1, beta2, eps, dt, fc))
+        self.set_max_iterations(max_iter)
+
+
+class Nesterov(_SimpleMinimizer):
+    """energyMinimizerNesterovAG on the device."""
+    _prefix = "qm_nesterov"
+
+    def __init__(self, dt=1e-4, mu=0.01, fc=1e-12, scheduled=False, max_iter=1000):
+        super().__init__()
+        _check(self.lib.qm_nesterov_set_parameters(self.h, dt, mu, fc, int(scheduled)))
+        self.set_max_iterations(max_iter)
+
+
+class LoLBFGS(_SimpleMinimizer):
+    """energyMinimizerLoLBFGS on the device."""
+    _prefix = "qm_lolbfgs"
+
+    def __init__(self, m=5, dt=1e-4, c=1.0, fc=1e-12, tau=10.0, scheduled=False, max_iter=1000):
+        super().__init__()
+        _check(self.lib.qm_lolbfgs_set_parameters(self.h, m, dt, c, fc, tau, int(scheduled)))
+        self.set_max_iterations(max_iter)
+
+
+def timing_enable(on=True):
+    _check(library().qm_timing_enable(int(on)))
+
+
+def timing_reset():
+    _check(library().qm_timing_reset())
+
+
+def timing_get(kernel):
+    n = ctypes.c_int64()
+    ms = ctypes.c_double()
+    _check(library().qm_timing_get(kernel, ctypes.byref(n), ctypes.byref(ms)))
+    return n.value, ms.value
+
+
+def device_count():
+    c = ctypes.c_int()
+    _check(library().qm_device_count(ctypes.byref(c)))
+    return c.value

@@ -1,0 +1,145 @@
+"""Generate HIP device code for the L2/L3/L4/L6 force stencils.
+
+Input: ldg_stencil_tables.json (per term and boundary case, entries
+[component, coefficient, input] or [component, coefficient, input_a, input_b]
+over the 125 per-site inputs: q[0..4], the 6 neighbours' Q [5..34] and their
+first-derivative vectors [35..124], neighbour order xd, xu, yd, yu, zd, zu and
+derivative order d/dx, d/dy, d/dz of q0..q4).
+
+Output: ../csrc/gen/ldg_stencils.inc with
+
+  ldg_bulk_lin_q(q, acc)                  centre-site part of the bulk L2/L3/L4 forms
+  ldg_bulk_lin_n<d>(nq, dA, dB, acc)      part of neighbour d (its Q and its two
+                                          transverse derivative vectors A < B)
+  ldg_bulk_l6(x, out)                     bulk L6 form (quadratic)
+  ldg_boundary_lin(bcase, x, acc)         the 27 boundary-case L2/L3/L4 forms
+  ldg_boundary_l6(bcase, x, out)          the 27 boundary-case L6 forms
+
+acc[t][c] accumulates term t (0 = L2, 1 = L3, 2 = L4) of component c at unit
+constant; the caller scales by L.  The bulk forms use no derivative along a
+neighbour's own axis; the generator checks that.
+"""
+import json
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TABLES = os.path.join(HERE, "ldg_stencil_tables.json")
+OUT = os.path.join(os.path.dirname(HERE), "csrc", "gen", "ldg_stencils.inc")
+LIN_TERMS = ("L2", "L3", "L4")
+
+
+def coef_literal(c):
+    r = repr(float(c))
+    if "e" not in r and "." not in r:
+        r += ".0"
+    return "R(%s)" % r
+
+
+def split_input(i):
+    """Classify input i: ('q', c) | ('n', d, c) | ('d', d, axis, c)."""
+    if i < 5:
+        return ("q", i)
+    if i < 35:
+        return ("n", (i - 5) // 5, (i - 5) % 5)
+    j = i - 35
+    return ("d", j // 15, (j % 15) // 5, j % 5)
+
+
+def emit_sum(lines, target, terms, var):
+    for coef, name in terms:
+        lines.append("    %s += %s * %s;" % (target, coef_literal(coef), name))
+
+
+def gen_bulk_linear(tables):
+    out = []
+    # centre
+    lines = ["template <typename R>",
+             "__device__ __forceinline__ void ldg_bulk_lin_q(const R* q, R (&acc)[3][5]) {"]
+    for t, name in enumerate(LIN_TERMS):
+        for e in tables[name]["bulk"]:
+            kind = split_input(e[2])
+            if kind[0] == "q":
+                lines.append("    acc[%d][%d] += %s * q[%d];" % (t, e[0], coef_literal(e[1]), kind[1]))
+    lines.append("}")
+    out += lines
+    for d in range(6):
+        axis = d // 2
+        trans = [a for a in range(3) if a != axis]
+        lines = ["template <typename R>",
+                 "__device__ __forceinline__ void ldg_bulk_lin_n%d(const R* nq, const R* dA, const R* dB, R (&acc)[3][5]) {" % d]
+        for t, name in enumerate(LIN_TERMS):
+            for e in tables[name]["bulk"]:
+                kind = split_input(e[2])
+                if kind[0] == "n" and kind[1] == d:
+                    lines.append("    acc[%d][%d] += %s * nq[%d];" % (t, e[0], coef_literal(e[1]), kind[2]))
+                elif kind[0] == "d" and kind[1] == d:
+                    assert kind[2] != axis, "bulk form uses a derivative along the neighbour's own axis"
+                    arr = "dA" if kind[2] == trans[0] else "dB"
+                    lines.append("    acc[%d][%d] += %s * %s[%d];" % (t, e[0], coef_literal(e[1]), arr, kind[3]))
+        lines.append("}")
+        out += lines
+    return out
+
+
+def xname(i):
+    return "x[%d]" % i
+
+
+def gen_l6(entries, fn_lines, indent="    "):
+    acc = {c: [] for c in range(5)}
+    for e in entries:
+        acc[e[0]].append(e)
+    for c in range(5):
+        fn_lines.append("%s{" % indent)
+        fn_lines.append("%s    R s = R(0);" % indent)
+        for e in acc[c]:
+            fn_lines.append("%s    s += %s * (%s * %s);" % (indent, coef_literal(e[1]), xname(e[2]), xname(e[3])))
+        fn_lines.append("%s    out[%d] = s;" % (indent, c))
+        fn_lines.append("%s}" % indent)
+
+
+def gen_boundary(tables, cases):
+    lines = ["template <typename R>",
+             "__device__ __forceinline__ void ldg_boundary_lin(int bcase, const R* x, R (&acc)[3][5]) {",
+             "    switch (bcase) {"]
+    for c in cases:
+        lines.append("    case %d: {" % c)
+        for t, name in enumerate(LIN_TERMS):
+            for e in tables[name][str(c)]:
+                lines.append("        acc[%d][%d] += %s * %s;" % (t, e[0], coef_literal(e[1]), xname(e[2])))
+        lines.append("        break;")
+        lines.append("    }")
+    lines += ["    default: break;", "    }", "}"]
+    l6 = ["template <typename R>",
+          "__device__ __forceinline__ void ldg_boundary_l6(int bcase, const R* x, R* out) {",
+          "    switch (bcase) {"]
+    for c in cases:
+        l6.append("    case %d: {" % c)
+        gen_l6(tables["L6"][str(c)], l6, "        ")
+        l6.append("        break;")
+        l6.append("    }")
+    l6 += ["    default: for (int i = 0; i < 5; ++i) out[i] = R(0); break;", "    }", "}"]
+    return lines + l6
+
+
+def main():
+    with open(TABLES) as fh:
+        doc = json.load(fh)
+    tables, cases = doc["tables"], doc["cases"]
+    body = ["// Generated by open-qmin_amd/stencils/gen_stencils.py from ldg_stencil_tables.json.",
+            "// Do not edit by hand.", "#pragma once", ""]
+    body += gen_bulk_linear(tables)
+    body += ["", "template <typename R>", "__device__ __forceinline__ void ldg_bulk_l6(const R* x, R* out) {"]
+    gen_l6(tables["L6"]["bulk"], body)
+    body += ["}", ""]
+    body += gen_boundary(tables, cases)
+    os.makedirs(os.path.dirname(OUT), exist_ok=True)
+    text = "\n".join(body) + "\n"
+    if not os.path.exists(OUT) or open(OUT).read() != text:
+        with open(OUT, "w") as fh:
+            fh.write(text)
+    print("wrote", OUT, len(text), "bytes")
+
+
+if __name__ == "__main__":
+    main()

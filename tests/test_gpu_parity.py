@@ -1,0 +1,160 @@
+"""HIP path (libqmin.so on the MI355X) against the CPU oracle on seeded inputs.
+
+Tolerances: FP64 forces within 1e-12 of the largest oracle force component on
+the lattice (the north-star bar is 1e-12 relative); FP64 energies within 1e-12
+relative; FIRE trajectories within 1e-10 relative per traced scalar (FP64
+reduction order differs from the oracle's serial sums); integer data bit-exact.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as po
+import qmin
+
+pytestmark = pytest.mark.gpu
+
+A, B, C = po.rescaled_phase()
+S0 = po.uniaxial_s0(A, B, C)
+
+ONE = (2.32, 0.0, 0.0, 0.0, 0.0)
+MULTI = (2.32, 0.5, 0.3, 0.0, 0.0)
+ALL = (2.32, 1.0, 0.5, 0.2, 0.3)
+
+
+def build_pair(shape, params, semantics, objects=True, efield=None, seed=1234, precision=qmin.QM_FP64):
+    o = po.OracleSystem(*shape)
+    o.random_init(seed, S0)
+    g = qmin.Lattice(*shape, precision=precision)
+    g.random_init(seed, S0)
+    if objects:
+        lx, ly, lz = shape
+        o.wall(2, 0, 1, 5.3, S0)
+        g.wall(2, 0, 1, 5.3, S0)
+        o.wall(2, lz - 1, 1, 5.3, S0)
+        g.wall(2, lz - 1, 1, 5.3, S0)
+        c = (lx / 2.0, ly / 2.0, lz / 2.0)
+        r = min(lx, ly, lz) / 4.0
+        o.sphere(c, r, 0, 5.3, S0)
+        g.sphere(c, r, 0, 5.3, S0)
+    o.set_params(A, B, C, *params, semantics=semantics)
+    f = qmin.LandauDeGennesLC(A, B, C, *params, semantics=semantics)
+    if efield is not None:
+        o.set_efield(efield, 1.0, 1.0, 1.0)
+        f.set_efield(efield, 1.0, 1.0, 1.0)
+    return o, g, f
+
+
+def assert_forces_close(got, ref, rel=1e-12):
+    scale = max(np.abs(ref).max(), 1e-300)
+    err = np.abs(got - ref).max()
+    assert err <= rel * scale, "max |dF| = %g, scale %g" % (err, scale)
+
+
+def test_lattice_setup_bit_exact():
+    shape = (13, 9, 11)
+    o, g, _ = build_pair(shape, MULTI, 0)
+    assert np.array_equal(g.types(), o.types())
+    assert np.array_equal(g.neighbors(), o.neighbors())
+    assert np.array_equal(g.q(), o.q()), "seeded initial Q must be bit-exact"
+    assert g.n_active() == o.n_active()
+
+
+@pytest.mark.parametrize("semantics", [0, 1])
+@pytest.mark.parametrize("params,objects,shape", [
+    (ONE, False, (12, 10, 9)),
+    (ONE, True, (35, 17, 12)),
+    (MULTI, False, (33, 9, 7)),
+    (MULTI, True, (40, 20, 16)),
+    (ALL, True, (24, 18, 14)),
+])
+def test_forces_fp64(semantics, params, objects, shape):
+    o, g, f = build_pair(shape, params, semantics, objects)
+    ref = o.forces()
+    got = f.compute_forces(g)
+    assert_forces_close(got, ref)
+
+
+def test_forces_with_field():
+    o, g, f = build_pair((20, 16, 18), ALL, 0, True, efield=(0.0, 0.0, 0.5))
+    assert_forces_close(f.compute_forces(g), o.forces())
+    o, g, f = build_pair((20, 16, 18), MULTI, 1, True, efield=(0.3, -0.2, 0.5))
+    assert_forces_close(f.compute_forces(g), o.forces())
+
+
+def test_forces_fp32_close():
+    o, g, f = build_pair((40, 20, 16), MULTI, 1, True, precision=qmin.QM_FP32)
+    ref = o.forces()
+    got = f.compute_forces(g)
+    assert_forces_close(got, ref, rel=2e-5)
+
+
+@pytest.mark.parametrize("params", [ONE, MULTI, ALL])
+def test_energy_fp64(params):
+    o, g, f = build_pair((24, 18, 14), params, 0, True, efield=(0.0, 0.1, 0.4))
+    e_ref, c_ref = o.energy()
+    e_got, c_got = f.compute_energy(g)
+    np.testing.assert_allclose(e_got, e_ref, rtol=1e-12)
+    np.testing.assert_allclose(c_got, c_ref, rtol=1e-11, atol=1e-12 * abs(e_ref))
+
+
+def test_defects():
+    o, g, _ = build_pair((16, 12, 10), MULTI, 0, True)
+    lc = o.types() <= 0
+    np.testing.assert_allclose(g.defects(0)[lc], o.defects(0)[lc], rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize("semantics", [0, 1])
+@pytest.mark.parametrize("params", [ONE, MULTI, ALL])
+def test_fire_trajectory_fp64(semantics, params):
+    shape = (20, 18, 12)
+    o, g, f = build_pair(shape, params, semantics, True)
+    it_ref, tr_ref = o.fire(max_iter=40)
+    m = qmin.FIRE(max_iter=40)
+    m.set_trace(64)
+    st = m.minimize(f, g)
+    assert st.iterations == it_ref == 40
+    tr = m.trace()
+    assert tr.shape == tr_ref.shape
+    np.testing.assert_array_equal(tr[:, 5], tr_ref[:, 5])       # N+ counter
+    np.testing.assert_allclose(tr[:, 0], tr_ref[:, 0], rtol=1e-10)  # forceMax
+    np.testing.assert_allclose(tr[:, 2:4], tr_ref[:, 2:4], rtol=1e-12)  # alpha, dt
+    np.testing.assert_allclose(tr[:, 1], tr_ref[:, 1], rtol=1e-9, atol=1e-12 * np.abs(tr_ref[:, 1]).max())
+    np.testing.assert_allclose(g.q(), o.q(), rtol=0, atol=1e-11)
+    np.testing.assert_allclose(g.v(), o.v(), rtol=0, atol=1e-11)
+
+
+def test_fire_one_constant_periodic_50():
+    """Config 1 of the workload set: 50^3 periodic, random start seed 1234, 100 FIRE steps."""
+    o, g, f = build_pair((50, 50, 50), ONE, 0, False)
+    it_ref, tr_ref = o.fire(max_iter=100)
+    m = qmin.FIRE(max_iter=100)
+    m.set_trace(128)
+    m.minimize(f, g)
+    tr = m.trace()
+    np.testing.assert_allclose(tr[:, 0], tr_ref[:, 0], rtol=1e-10)
+    np.testing.assert_allclose(g.q(), o.q(), rtol=0, atol=1e-11)
+    e_ref, _ = o.energy()
+    e_got, _ = f.compute_energy(g)
+    np.testing.assert_allclose(e_got, e_ref, rtol=1e-12)
+
+
+def test_fire_stops_on_tolerance_like_reference():
+    o, g, f = build_pair((16, 12, 10), ONE, 0, True)
+    it_ref, tr_ref = o.fire(fc=0.02, max_iter=5000)
+    m = qmin.FIRE(fc=0.02, max_iter=5000)
+    st = m.minimize(f, g)
+    assert st.iterations == it_ref
+    assert st.converged and st.forceMax <= 0.02
+    np.testing.assert_allclose(st.forceMax, tr_ref[-1, 0], rtol=1e-10)
+
+
+def test_fire_continues_across_calls():
+    """Two minimize calls of 20 iterations equal one of 40 (state persists on the device)."""
+    o, g, f = build_pair((18, 14, 12), MULTI, 0, True)
+    o.fire(max_iter=40)
+    m = qmin.FIRE(max_iter=20)
+    m.minimize(f, g)
+    m.set_max_iterations(40)
+    m.minimize(f, g)
+    assert m.state().iterations == 40
+    np.testing.assert_allclose(g.q(), o.q(), rtol=0, atol=1e-11)
