@@ -20,9 +20,10 @@
 // scalars on the device (energyMinimizerFIRE.cpp:77-145): no host round trip.
 #include "qmin_device.h"
 #include "ldg_math.h"
-#include "gen/ldg_stencils.inc"
 
 namespace qmin {
+
+#include "gen/ldg_stencils.inc"
 
 __device__ __forceinline__ int wrapi(int v, int m) {
     v %= m;
@@ -83,145 +84,71 @@ __device__ __forceinline__ void nbr_derivative(int tn, int tu, int td, const R* 
     }
 }
 
+// Central-difference derivatives of the six face neighbours of the LDS centre
+// h (slot sc), transverse axes only, in the layout of the stencil tables'
+// derivative inputs x[35 + 15 d + 5 axis + c] (along-axis entries zero).
 template <typename R>
-struct Tile {
-    R (*q)[5][kHP];
-    int8_t (*t)[kHP];
-    __device__ __forceinline__ void load(int s, int h, R* v) const {
+__device__ __forceinline__ void central_nbr_derivatives(const R (*q)[5][kHP], int sm, int sc, int sp, int h, R* x) {
 #pragma unroll
-        for (int c = 0; c < 5; ++c) v[c] = q[s][c][h];
-    }
-};
-
-// Neighbour d of the centre (slot sc, index h) and, for each of its two
-// transverse axes A < B, the positions of n + e_axis and n - e_axis.
-struct NbrGeom {
-    int sn, hn;
-    int suA, huA, sdA, hdA;
-    int suB, huB, sdB, hdB;
-};
-
-__device__ __forceinline__ NbrGeom nbr_geom(int d, int sm, int sc, int sp, int h) {
-    NbrGeom g;
-    switch (d) {
-        case 0:
-        case 1: {
-            const int hn = d == 0 ? h - 1 : h + 1;
-            g = {sc, hn, sc, hn + kHX, sc, hn - kHX, sp, hn, sm, hn};
-            break;
-        }
-        case 2:
-        case 3: {
-            const int hn = d == 2 ? h - kHX : h + kHX;
-            g = {sc, hn, sc, hn + 1, sc, hn - 1, sp, hn, sm, hn};
-            break;
-        }
-        default: {
-            const int sn = d == 4 ? sm : sp;
-            g = {sn, h, sn, h + 1, sn, h - 1, sn, h + kHX, sn, h - kHX};
-            break;
-        }
-    }
-    return g;
-}
-
-template <typename R, int D>
-__device__ __forceinline__ void bulk_lin_nbr(const R* nq, const R* dA, const R* dB, R (&acc)[3][5]) {
-    if (D == 0) ldg_bulk_lin_n0(nq, dA, dB, acc);
-    if (D == 1) ldg_bulk_lin_n1(nq, dA, dB, acc);
-    if (D == 2) ldg_bulk_lin_n2(nq, dA, dB, acc);
-    if (D == 3) ldg_bulk_lin_n3(nq, dA, dB, acc);
-    if (D == 4) ldg_bulk_lin_n4(nq, dA, dB, acc);
-    if (D == 5) ldg_bulk_lin_n5(nq, dA, dB, acc);
-}
-
-template <typename R, int TERMS, int D>
-__device__ __forceinline__ void bulk_neighbour(const Tile<R>& T, int sm, int sc, int sp, int h, R (&acc)[3][5],
-                                               R* x) {
-    const NbrGeom g = nbr_geom(D, sm, sc, sp, h);
-    R nq[5], qu[5], qd[5], dA[5], dB[5];
-    T.load(g.sn, g.hn, nq);
-    const int tn = T.t[g.sn][g.hn];
-    T.load(g.suA, g.huA, qu);
-    T.load(g.sdA, g.hdA, qd);
-    nbr_derivative(tn, int(T.t[g.suA][g.huA]), int(T.t[g.sdA][g.hdA]), nq, qu, qd, dA);
-    T.load(g.suB, g.huB, qu);
-    T.load(g.sdB, g.hdB, qd);
-    nbr_derivative(tn, int(T.t[g.suB][g.huB]), int(T.t[g.sdB][g.hdB]), nq, qu, qd, dB);
-    bulk_lin_nbr<R, D>(nq, dA, dB, acc);
-    if constexpr (TERMS == kMultiL6) {
-        // x layout of the stencil tables: q[5], neighbour Q [6][5], derivatives [6][3][5]
-        const int axis = D / 2, a = axis == 0 ? 1 : 0, b = axis == 2 ? 1 : 2;
+    for (int d = 0; d < 6; ++d) {
+        const int axis = d / 2;
+        const int sn = d == 4 ? sm : (d == 5 ? sp : sc);
+        const int hn = d == 0 ? h - 1 : d == 1 ? h + 1 : d == 2 ? h - kHX : d == 3 ? h + kHX : h;
 #pragma unroll
-        for (int c = 0; c < 5; ++c) {
-            x[5 + 5 * D + c] = nq[c];
-            x[35 + 15 * D + 5 * a + c] = dA[c];
-            x[35 + 15 * D + 5 * b + c] = dB[c];
-            x[35 + 15 * D + 5 * axis + c] = R(0);
+        for (int ax = 0; ax < 3; ++ax) {
+            R* out = x + 35 + 15 * d + 5 * ax;
+            if (ax == axis) {
+#pragma unroll
+                for (int c = 0; c < 5; ++c) out[c] = R(0);
+                continue;
+            }
+            int su = sn, sd = sn, hu = hn, hd = hn;
+            if (ax == 0) { hu = hn + 1; hd = hn - 1; }
+            if (ax == 1) { hu = hn + kHX; hd = hn - kHX; }
+            if (ax == 2) { su = sp; sd = sm; }
+#pragma unroll
+            for (int c = 0; c < 5; ++c) out[c] = R(0.5) * (q[su][c][hu] - q[sd][c][hd]);
         }
+#pragma unroll
+        for (int c = 0; c < 5; ++c) x[5 + 5 * d + c] = q[sn][c][hn];
     }
 }
 
-// Raw force (phase - distortion) of a t == 0 site at LDS centre h of slot sc.
+// Raw force (phase - distortion) of a site whose neighbours all have central
+// derivatives: the bulk L1..L4 forms composed with those differences into one
+// linear 19-point operator (weights W), plus the quadratic L6 form.
+// Uniform read-only data seen through the constant address space, so that the
+// compiler fetches it with scalar loads.
+template <typename R>
+using ConstPtr = const R __attribute__((address_space(4)))*;
+
 template <typename R, int TERMS>
-__device__ __forceinline__ void bulk_raw_force(const ForceConst& fc, const Tile<R>& T, int sm, int sc, int sp, int h,
-                                               const R* q, R* F) {
+__device__ __forceinline__ void fast_raw_force(const ForceConst& fc, ConstPtr<R> W, const R (*q)[5][kHP],
+                                               int sm, int sc, int sp, int h, const R* qc, R* F) {
 #pragma unroll
     for (int c = 0; c < 5; ++c) F[c] = R(0);
-    phase_force(R(fc.a), R(fc.b), R(fc.c), q, F);
-    R spatial[5];
-    {
-        R nb[5], lap[5];
+    phase_force(R(fc.a), R(fc.b), R(fc.c), qc, F);
+    R spatial[5] = {R(0), R(0), R(0), R(0), R(0)};
+    if constexpr (TERMS == kOneConstant) {
 #pragma unroll
-        for (int c = 0; c < 5; ++c) lap[c] = R(6) * q[c];
-        T.load(sc, h - 1, nb);
-#pragma unroll
-        for (int c = 0; c < 5; ++c) lap[c] -= nb[c];
-        T.load(sc, h + 1, nb);
-#pragma unroll
-        for (int c = 0; c < 5; ++c) lap[c] -= nb[c];
-        T.load(sc, h - kHX, nb);
-#pragma unroll
-        for (int c = 0; c < 5; ++c) lap[c] -= nb[c];
-        T.load(sc, h + kHX, nb);
-#pragma unroll
-        for (int c = 0; c < 5; ++c) lap[c] -= nb[c];
-        T.load(sm, h, nb);
-#pragma unroll
-        for (int c = 0; c < 5; ++c) lap[c] -= nb[c];
-        T.load(sp, h, nb);
-#pragma unroll
-        for (int c = 0; c < 5; ++c) spatial[c] = R(fc.L1) * (lap[c] - nb[c]);
+        for (int c = 0; c < 5; ++c)
+            spatial[c] = R(fc.L1) * (R(6) * qc[c] - q[sc][c][h - 1] - q[sc][c][h + 1] - q[sc][c][h - kHX] -
+                                     q[sc][c][h + kHX] - q[sm][c][h] - q[sp][c][h]);
         l1_shape(spatial);
+    } else if constexpr (TERMS == kMultiLinear) {
+        ldg_fast_lin_a(W, q, sm, sc, sp, h, spatial);
+    } else {
+        ldg_fast_lin_b(W, q, sm, sc, sp, h, spatial);
     }
-    if constexpr (TERMS != kOneConstant) {
-        R acc[3][5];
+    if constexpr (TERMS == kMultiL6) {
+        R x[125];
 #pragma unroll
-        for (int t = 0; t < 3; ++t)
+        for (int c = 0; c < 5; ++c) x[c] = qc[c];
+        central_nbr_derivatives(q, sm, sc, sp, h, x);
+        R o6[5];
+        ldg_bulk_l6(x, o6);
 #pragma unroll
-            for (int c = 0; c < 5; ++c) acc[t][c] = R(0);
-        R x[TERMS == kMultiL6 ? 125 : 1];
-        ldg_bulk_lin_q(q, acc);
-        bulk_neighbour<R, TERMS, 0>(T, sm, sc, sp, h, acc, x);
-        bulk_neighbour<R, TERMS, 1>(T, sm, sc, sp, h, acc, x);
-        bulk_neighbour<R, TERMS, 2>(T, sm, sc, sp, h, acc, x);
-        bulk_neighbour<R, TERMS, 3>(T, sm, sc, sp, h, acc, x);
-        bulk_neighbour<R, TERMS, 4>(T, sm, sc, sp, h, acc, x);
-        bulk_neighbour<R, TERMS, 5>(T, sm, sc, sp, h, acc, x);
-#pragma unroll
-        for (int c = 0; c < 5; ++c) spatial[c] += R(fc.L2) * acc[0][c];
-#pragma unroll
-        for (int c = 0; c < 5; ++c) spatial[c] += R(fc.L3) * acc[1][c];
-#pragma unroll
-        for (int c = 0; c < 5; ++c) spatial[c] += R(fc.L4) * acc[2][c];
-        if constexpr (TERMS == kMultiL6) {
-#pragma unroll
-            for (int c = 0; c < 5; ++c) x[c] = q[c];
-            R o6[5];
-            ldg_bulk_l6(x, o6);
-#pragma unroll
-            for (int c = 0; c < 5; ++c) spatial[c] += R(fc.L6) * o6[c];
-        }
+        for (int c = 0; c < 5; ++c) spatial[c] += R(fc.L6) * o6[c];
     }
 #pragma unroll
     for (int c = 0; c < 5; ++c) F[c] -= spatial[c];
@@ -240,7 +167,6 @@ __device__ __forceinline__ void finish_bulk(const ForceConst& fc, R* F) {
 template <typename R, int MODE, int TERMS>
 __global__ void __launch_bounds__(kThreads) march_kernel(MarchArgs<R> A) {
     __shared__ R sQ[3][5][kHP];
-    __shared__ int8_t sT[3][kHP];
     __shared__ double sRed[kThreads / 64][3];
 
     Ctrl* ctrl = A.ctrl;
@@ -248,16 +174,17 @@ __global__ void __launch_bounds__(kThreads) march_kernel(MarchArgs<R> A) {
     const int cur = ctrl->cur;
     const StateSet<R> in = A.set[cur];
     const StateSet<R> out = A.set[MODE == kFire ? cur ^ 1 : cur];
+    // pending FIRE velocity mix v = ck u + cf F: (1, 0) when none is pending,
+    // (0, 0) after a reset (energyMinimizerFIRE.cpp:117-144)
     R dt = 0, hdt = 0, hdt2 = 0, ck = 0, cf = 0;
-    int mix = 0;  // 0 v = u, 1 v = ck u + cf F, 2 v = 0
     if (MODE == kFire) {
         const double dtd = ctrl->dt;
         dt = R(dtd);
         hdt = R(0.5 * dtd);
         hdt2 = R(0.5 * dtd * dtd);
-        ck = R(ctrl->cKeep);
-        cf = R(ctrl->cForce);
-        mix = ctrl->reset ? 2 : (ctrl->identity ? 0 : 1);
+        const int reset = ctrl->reset, identity = ctrl->identity;
+        ck = reset ? R(0) : (identity ? R(1) : R(ctrl->cKeep));
+        cf = (reset || identity) ? R(0) : R(ctrl->cForce);
     }
 
     // XCD-aware tile order: consecutive work-group ids go round-robin over the
@@ -298,14 +225,13 @@ __global__ void __launch_bounds__(kThreads) march_kernel(MarchArgs<R> A) {
         colHalo = wrapi(txi * kTX - 1 + hx, lx) + (long long)lx * wrapi(tyi * kTY - 1 + hy, ly);
     }
 
-    const Tile<R> T{sQ, sT};
-
-    // Stage plane z into slot s; returns the owner's half-kicked velocity.
-    auto stage = [&](int z, int s, R* vh, int& tOwn) {
+    // Stage plane z into slot s (FIRE: pending mix, drift and first half kick);
+    // returns the owner's half-kicked velocity and class.
+    auto stage = [&](int z, int s, R* vh, int& cOwn) {
         const long long zoff = plane * wrapi(z, lz);
         {
             const long long i = colOwn + zoff;
-            const int8_t t = A.types[i];
+            cOwn = A.cls[i];
             R q[5];
 #pragma unroll
             for (int c = 0; c < 5; ++c) q[c] = in.q[c * n + i];
@@ -318,15 +244,13 @@ __global__ void __launch_bounds__(kThreads) march_kernel(MarchArgs<R> A) {
                 }
 #pragma unroll
                 for (int c = 0; c < 5; ++c) {
-                    const R v = mix == 2 ? R(0) : (mix == 0 ? u[c] : ck * u[c] + cf * f[c]);
+                    const R v = ck * u[c] + cf * f[c];
                     q[c] = q[c] + (dt * v + hdt2 * f[c]);
                     vh[c] = v + hdt * f[c];
                 }
             }
 #pragma unroll
             for (int c = 0; c < 5; ++c) sQ[s][c][hOwn] = q[c];
-            sT[s][hOwn] = t;
-            tOwn = t;
         }
         if (hHalo >= 0) {
             const long long i = colHalo + zoff;
@@ -334,36 +258,45 @@ __global__ void __launch_bounds__(kThreads) march_kernel(MarchArgs<R> A) {
 #pragma unroll
             for (int c = 0; c < 5; ++c) q[c] = in.q[c * n + i];
             if (MODE == kFire) {
+                R u[5], f[5];
 #pragma unroll
                 for (int c = 0; c < 5; ++c) {
-                    const R u = in.u[c * n + i], f = in.f[c * n + i];
-                    const R v = mix == 2 ? R(0) : (mix == 0 ? u : ck * u + cf * f);
-                    q[c] = q[c] + (dt * v + hdt2 * f);
+                    u[c] = in.u[c * n + i];
+                    f[c] = in.f[c * n + i];
+                }
+#pragma unroll
+                for (int c = 0; c < 5; ++c) {
+                    const R v = ck * u[c] + cf * f[c];
+                    q[c] = q[c] + (dt * v + hdt2 * f[c]);
                 }
             }
 #pragma unroll
             for (int c = 0; c < 5; ++c) sQ[s][c][hHalo] = q[c];
-            sT[s][hHalo] = A.types[i];
         }
     };
 
     double pF = 0, pP = 0, pV = 0;
     R vhCur[5], vhNext[5];
-    int tCur = 0, tNext = 0;
-    stage(z0 - 1, 0, vhNext, tNext);
-    stage(z0, 1, vhCur, tCur);
+    int cCur = 0, cNext = 0;
+    stage(z0 - 1, 0, vhNext, cNext);
+    stage(z0, 1, vhCur, cCur);
     for (int z = z0; z < z1; ++z) {
         const int sm = (z - z0) % 3, sc = (z - z0 + 1) % 3, sp = (z - z0 + 2) % 3;
-        stage(z + 1, sp, vhNext, tNext);
+        stage(z + 1, sp, vhNext, cNext);
         __syncthreads();
         if (active) {
             const long long i = colOwn + plane * z;
             R q[5];
-            T.load(sc, hOwn, q);
-            if (tCur >= 0) {
+#pragma unroll
+            for (int c = 0; c < 5; ++c) q[c] = sQ[sc][c][hOwn];
+            if (cCur >= 0) {
                 R F[5];
-                if (tCur == 0) {
-                    bulk_raw_force<R, TERMS>(A.fc, T, sm, sc, sp, hOwn, q, F);
+                if (cCur == 0) {
+                    // re-opaque the weight pointer every step: the loads stay
+                    // inside the loop instead of being hoisted and spilled
+                    uint64_t wa = reinterpret_cast<uint64_t>(A.W);
+                    asm volatile("" : "+s"(wa));
+                    fast_raw_force<R, TERMS>(A.fc, reinterpret_cast<ConstPtr<R>>(wa), sQ, sm, sc, sp, hOwn, q, F);
                     finish_bulk(A.fc, F);
                 } else {
 #pragma unroll
@@ -398,7 +331,7 @@ __global__ void __launch_bounds__(kThreads) march_kernel(MarchArgs<R> A) {
         __syncthreads();
 #pragma unroll
         for (int c = 0; c < 5; ++c) vhCur[c] = vhNext[c];
-        tCur = tNext;
+        cCur = cNext;
     }
     if (MODE == kFire) {
         block_sum3<kThreads>(pF, pP, pV, sRed);
@@ -592,14 +525,19 @@ __global__ void __launch_bounds__(kBoundaryThreads) boundary_kernel(BoundaryArgs
         }
 #pragma unroll
         for (int c = 0; c < 5; ++c) F[c] -= spatial[c];
+        if (self == 0) {
+            // a bulk site with a one-sided neighbour derivative: bulk semantics
+            finish_bulk(A.fc, F);
+        } else {
 #pragma unroll
-        for (int d = 0; d < 6; ++d)
-            if (tn[d] > 0) add_anchoring_force(A.objects[tn[d] - 1], q, xin + 5 + 5 * d, F);
-        if (A.fc.hasField) {
+            for (int d = 0; d < 6; ++d)
+                if (tn[d] > 0) add_anchoring_force(A.objects[tn[d] - 1], q, xin + 5 + 5 * d, F);
+            if (A.fc.hasField) {
 #pragma unroll
-            for (int c = 0; c < 5; ++c) F[c] += R(A.fc.field[c]);
+                for (int c = 0; c < 5; ++c) F[c] += R(A.fc.field[c]);
+            }
+            metric(F);
         }
-        metric(F);
         if (MODE == kFire) {
             R u[5];
             gload(st.u, n, i, u);
@@ -793,10 +731,12 @@ void launch_march(const MarchArgs<R>& a, int mode, int terms, hipStream_t s) {
     if (mode == kFire) {
         if (terms == kOneConstant) launch_march_t<R, kFire, kOneConstant>(a, s);
         else if (terms == kMultiLinear) launch_march_t<R, kFire, kMultiLinear>(a, s);
+        else if (terms == kMultiL4) launch_march_t<R, kFire, kMultiL4>(a, s);
         else launch_march_t<R, kFire, kMultiL6>(a, s);
     } else {
         if (terms == kOneConstant) launch_march_t<R, kForce, kOneConstant>(a, s);
         else if (terms == kMultiLinear) launch_march_t<R, kForce, kMultiLinear>(a, s);
+        else if (terms == kMultiL4) launch_march_t<R, kForce, kMultiL4>(a, s);
         else launch_march_t<R, kForce, kMultiL6>(a, s);
     }
 }
@@ -805,15 +745,16 @@ template <typename R>
 void launch_boundary(const BoundaryArgs<R>& a, int mode, int terms, hipStream_t s) {
     long long blocks = (a.nsites + kBoundaryThreads - 1) / kBoundaryThreads;
     if (blocks < 1) blocks = 1;
+    // the boundary pass does not distinguish L1..L3 from L1..L4
     if (mode == kFire) {
         if (terms == kOneConstant) launch_boundary_t<R, kFire, kOneConstant>(a, int(blocks), s);
-        else if (terms == kMultiLinear) launch_boundary_t<R, kFire, kMultiLinear>(a, int(blocks), s);
-        else launch_boundary_t<R, kFire, kMultiL6>(a, int(blocks), s);
+        else if (terms == kMultiL6) launch_boundary_t<R, kFire, kMultiL6>(a, int(blocks), s);
+        else launch_boundary_t<R, kFire, kMultiLinear>(a, int(blocks), s);
     } else {
         if (a.nsites == 0) return;
         if (terms == kOneConstant) launch_boundary_t<R, kForce, kOneConstant>(a, int(blocks), s);
-        else if (terms == kMultiLinear) launch_boundary_t<R, kForce, kMultiLinear>(a, int(blocks), s);
-        else launch_boundary_t<R, kForce, kMultiL6>(a, int(blocks), s);
+        else if (terms == kMultiL6) launch_boundary_t<R, kForce, kMultiL6>(a, int(blocks), s);
+        else launch_boundary_t<R, kForce, kMultiLinear>(a, int(blocks), s);
     }
 }
 

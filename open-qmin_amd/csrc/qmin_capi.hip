@@ -16,6 +16,7 @@
 #include "../../include/qmin.h"
 #include "qmin_device.h"
 #include "host_init.h"
+#include "gen/ldg_fast_host.inc"
 
 namespace qmin {
 template <typename R>
@@ -77,9 +78,11 @@ struct qm_lattice {
     std::vector<int32_t> types;      // host copy, reference encoding
     std::vector<HostObject> objects;
     int8_t* dTypes = nullptr;
+    int8_t* dClass = nullptr;             // 0 composed bulk formula, > 0 object, < 0 boundary pass
+    void* dW = nullptr;                   // composed bulk operator weights (Real)
     DevObject* dObjects = nullptr;
     void* dSet[2] = {nullptr, nullptr};   // each 15 n reals: Q, U, F planes
-    int32_t* dSites = nullptr;            // compacted t < 0 sites
+    int32_t* dSites = nullptr;            // compacted sites finished by the boundary pass
     long long nBoundary = 0;
     Ctrl* dCtrl = nullptr;
     Ctrl hCtrl{};                          // host mirror, valid after syncCtrl
@@ -155,16 +158,44 @@ static int push_ctrl(qm_lattice* L) {
     return QM_OK;
 }
 
-// Upload types/objects and rebuild the compacted boundary-site list.
+// Upload types/objects, classify sites for the fused pass and rebuild the
+// compacted list of sites finished by the boundary pass: t < 0 sites, and t == 0
+// sites with a neighbour whose derivative is one-sided (an object two steps away
+// along an edge diagonal, landauDeGennesLCOtherForces.cpp:77-129).  Every other
+// t == 0 site has central derivatives at all six neighbours, where the composed
+// bulk operator is exact.
 static int refresh_types(qm_lattice* L) {
     if (!L->typesDirty) return QM_OK;
-    std::vector<int8_t> t8(L->n);
+    const long long n = L->n;
+    const int lx = L->lx, ly = L->ly, lz = L->lz;
+    std::vector<int8_t> t8(n), cls(n);
     std::vector<int32_t> sites;
-    for (long long i = 0; i < L->n; ++i) {
-        t8[i] = (int8_t)L->types[i];
-        if (L->types[i] < 0) sites.push_back((int32_t)i);
-    }
-    HIP_TRY(hipMemcpy(L->dTypes, t8.data(), L->n, hipMemcpyHostToDevice));
+    auto T = [&](int x, int y, int z) { return L->types[site_of(L, wrapc(x, lx), wrapc(y, ly), wrapc(z, lz))]; };
+    for (int z = 0; z < lz; ++z)
+        for (int y = 0; y < ly; ++y)
+            for (int x = 0; x < lx; ++x) {
+                const int64_t i = site_of(L, x, y, z);
+                const int t = L->types[i];
+                t8[i] = (int8_t)t;
+                int c = t > 0 ? 1 : (t < 0 ? -1 : 0);
+                if (t == 0) {
+                    static const int dir[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+                    for (int d = 0; d < 6 && c == 0; ++d) {
+                        const int nx = x + dir[d][0], ny = y + dir[d][1], nz = z + dir[d][2];
+                        const int tn = T(nx, ny, nz);
+                        if (tn > 0) c = -1;
+                        if (tn >= 0) continue;
+                        for (int e = 0; e < 6 && c == 0; ++e) {
+                            if (e / 2 == d / 2) continue;
+                            if (T(nx + dir[e][0], ny + dir[e][1], nz + dir[e][2]) > 0) c = -1;
+                        }
+                    }
+                }
+                cls[i] = (int8_t)c;
+                if (c < 0) sites.push_back((int32_t)i);
+            }
+    HIP_TRY(hipMemcpy(L->dTypes, t8.data(), n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(L->dClass, cls.data(), n, hipMemcpyHostToDevice));
     std::vector<DevObject> objs(kMaxObjects);
     for (size_t j = 0; j < L->objects.size(); ++j) objs[j] = {L->objects[j].type, 0, L->objects[j].W, L->objects[j].S0};
     HIP_TRY(hipMemcpy(L->dObjects, objs.data(), sizeof(DevObject) * kMaxObjects, hipMemcpyHostToDevice));
@@ -242,6 +273,8 @@ int qm_lattice_create(int lx, int ly, int lz, int precision, qm_lattice** out) {
     L->types.assign(n, 0);
     HIP_TRY(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking));
     HIP_TRY(hipMalloc(&L->dTypes, n));
+    HIP_TRY(hipMalloc(&L->dClass, n));
+    HIP_TRY(hipMalloc(&L->dW, sizeof(double) * kFastWeights));
     HIP_TRY(hipMalloc(&L->dObjects, sizeof(DevObject) * kMaxObjects));
     for (int s = 0; s < 2; ++s) {
         HIP_TRY(hipMalloc(&L->dSet[s], 15 * n * L->realSize));
@@ -260,6 +293,8 @@ int qm_lattice_destroy(qm_lattice* L) {
     if (!L) return QM_OK;
     (void)hipStreamSynchronize(L->stream);
     (void)hipFree(L->dTypes);
+    (void)hipFree(L->dClass);
+    (void)hipFree(L->dW);
     (void)hipFree(L->dObjects);
     (void)hipFree(L->dSet[0]);
     (void)hipFree(L->dSet[1]);
@@ -617,14 +652,47 @@ static ForceConst force_const(const qm_ldg* f) {
 
 static int terms_of(const qm_ldg* f) {
     if (!f->multi) return kOneConstant;
-    return f->L6 != 0 ? kMultiL6 : kMultiLinear;
+    if (f->L6 != 0) return kMultiL6;
+    return f->L4 != 0 ? kMultiL4 : kMultiLinear;
+}
+
+// Weights of the composed bulk operator for the force's constants.
+template <typename R>
+static void composed_weights(const qm_ldg* f, R* w) {
+    for (int k = 0; k < kFastWeights; ++k) w[k] = R(0);
+    const int terms = terms_of(f);
+    if (terms == kOneConstant) return;
+    const double Ls[4] = {f->L1, f->L2, f->L3, f->L4};
+    const bool b = terms != kMultiLinear;
+    const int cnt = b ? kFastLinBCount : kFastLinACount;
+    for (int k = 0; k < cnt; ++k) {
+        const double* c = b ? kFastLinB[k] : kFastLinA[k];
+        double acc = 0;
+        for (int t = 0; t < 4; ++t) acc += Ls[t] * c[t];
+        w[k] = R(acc);
+    }
+}
+static_assert(kFastLinBCount <= kFastWeights, "composed operator larger than the weight buffer");
+
+static int upload_weights(qm_lattice* L, const qm_ldg* f) {
+    if (L->precision == QM_FP64) {
+        double w[kFastWeights];
+        composed_weights<double>(f, w);
+        HIP_TRY(hipMemcpy(L->dW, w, sizeof(w), hipMemcpyHostToDevice));
+    } else {
+        float w[kFastWeights];
+        composed_weights<float>(f, w);
+        HIP_TRY(hipMemcpy(L->dW, w, sizeof(w), hipMemcpyHostToDevice));
+    }
+    return QM_OK;
 }
 
 template <typename R>
 static void fill_args(qm_lattice* L, const qm_ldg* f, MarchArgs<R>& ma, BoundaryArgs<R>& ba, const FireConst& fire) {
     ma.set[0] = state_set<R>(L, 0);
     ma.set[1] = state_set<R>(L, 1);
-    ma.types = L->dTypes;
+    ma.cls = L->dClass;
+    ma.W = static_cast<const R*>(L->dW);
     ma.lx = L->lx;
     ma.ly = L->ly;
     ma.lz = L->lz;
@@ -695,6 +763,7 @@ static int run_pass(qm_lattice* L, const qm_ldg* f, int mode, const FireConst& f
 static int prepare(qm_lattice* L, const qm_ldg* f) {
     if (!L || !f) return fail(QM_EINVAL, "null handle");
     if (int rc = refresh_types(L)) return rc;
+    if (int rc = upload_weights(L, f)) return rc;
     const size_t need = 3 * (size_t)(march_blocks(L) + boundary_blocks(L->nBoundary));
     return ensure_partials(L, need);
 }

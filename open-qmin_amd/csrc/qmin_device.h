@@ -18,9 +18,11 @@ constexpr int kMaxObjects = 127;        // types are int8 on the device
 constexpr int kBoundaryThreads = 128;
 constexpr int kReduceThreads = 256;
 constexpr int kMaxEnergyBlocks = 4096;
+constexpr int kFastWeights = 160;  // >= entries of the composed L1..L4 operator
 
 enum Mode { kForce = 0, kFire = 1 };
-enum Terms { kOneConstant = 0, kMultiLinear = 1, kMultiL6 = 2 };
+// kMultiLinear: L1..L3, kMultiL4: L1..L4, kMultiL6: L1..L4 and L6
+enum Terms { kOneConstant = 0, kMultiLinear = 1, kMultiL4 = 2, kMultiL6 = 3 };
 
 struct DevObject {
     int type;  // 0 homeotropic, 1 degenerate planar
@@ -80,13 +82,14 @@ struct StateSet {
 template <typename R>
 struct MarchArgs {
     StateSet<R> set[2];
-    const int8_t* types;
+    const int8_t* cls;      // per site: 0 composed bulk formula, > 0 object, < 0 finished by the boundary pass
     int lx, ly, lz, zchunk;
     long long n;
     int nbx, nby, nblocks;
     ForceConst fc;
     Ctrl* ctrl;
     double* partials;   // [nblocks + boundary blocks][3]
+    const R* W;         // composed bulk operator weights (device buffer, read with scalar loads)
 };
 
 template <typename R>

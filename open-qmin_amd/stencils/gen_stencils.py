@@ -18,6 +18,15 @@ Output: ../csrc/gen/ldg_stencils.inc with
 acc[t][c] accumulates term t (0 = L2, 1 = L3, 2 = L4) of component c at unit
 constant; the caller scales by L.  The bulk forms use no derivative along a
 neighbour's own axis; the generator checks that.
+
+It also composes, for a site whose neighbours all have central derivatives,
+the bulk L1 (shaped Laplacian), L2, L3 and L4 forms with those central
+differences into one linear operator on the 19-point neighbourhood (centre,
+6 faces, 12 edges).  Its weights are L1 C1 + L2 C2 + L3 C3 + L4 C4 per entry:
+
+  ldg_fast_lin_a / _b(W, q, sm, sc, sp, h, acc)   device: acc[r] += W[k] Q_c(p)
+  ../csrc/gen/ldg_fast_host.inc                   host: C1..C4 per entry k
+variant a covers L1..L3 (L4 = 0), variant b L1..L4.
 """
 import json
 import os
@@ -122,6 +131,102 @@ def gen_boundary(tables, cases):
     return lines + l6
 
 
+POINTS = [(0, 0, 0), (-1, 0, 0), (1, 0, 0), (0, -1, 0), (0, 1, 0), (0, 0, -1), (0, 0, 1)]
+POINTS += [(a, b, 0) for a in (-1, 1) for b in (-1, 1)] + [(a, 0, b) for a in (-1, 1) for b in (-1, 1)]
+POINTS += [(0, a, b) for a in (-1, 1) for b in (-1, 1)]
+NBR = [(-1, 0, 0), (1, 0, 0), (0, -1, 0), (0, 1, 0), (0, 0, -1), (0, 0, 1)]
+AXES = [(1, 0, 0), (0, 1, 0), (0, 0, 1)]
+
+
+def compose(entries):
+    """Bulk table over (q, neighbour Q, neighbour derivatives) -> {(r, point, comp): coef}
+    with the neighbour derivatives replaced by central differences."""
+    from fractions import Fraction
+    out = {}
+
+    def add(r, p, c, v):
+        key = (r, p, c)
+        out[key] = out.get(key, Fraction(0)) + v
+
+    for e in entries:
+        r, coef, i = e[0], Fraction(e[1]), e[2]
+        if i < 5:
+            add(r, (0, 0, 0), i, coef)
+        elif i < 35:
+            d, c = (i - 5) // 5, (i - 5) % 5
+            add(r, NBR[d], c, coef)
+        else:
+            j = i - 35
+            d, a, c = j // 15, (j % 15) // 5, j % 5
+            n, ax = NBR[d], AXES[a]
+            up = tuple(n[k] + ax[k] for k in range(3))
+            dn = tuple(n[k] - ax[k] for k in range(3))
+            add(r, up, c, coef / 2)
+            add(r, dn, c, -coef / 2)
+    return {k: v for k, v in out.items() if v != 0}
+
+
+def shaped_laplacian():
+    """bulkL1Force at L1 = 1: s = 6 q - sum of faces, then s0 += s0 + s3,
+    s3 += s0 + s3, s1, s2, s4 doubled."""
+    from fractions import Fraction
+    raw = {}
+    for c in range(5):
+        raw[(c, (0, 0, 0), c)] = Fraction(6)
+        for p in NBR:
+            raw[(c, p, c)] = Fraction(-1)
+    shape = {0: {0: 2, 3: 1}, 1: {1: 2}, 2: {2: 2}, 3: {0: 1, 3: 2}, 4: {4: 2}}
+    out = {}
+    for (s, p, c), v in raw.items():
+        for r, row in shape.items():
+            if s in row:
+                out[(r, p, c)] = out.get((r, p, c), Fraction(0)) + row[s] * v
+    return {k: v for k, v in out.items() if v != 0}
+
+
+FENCE_EVERY = int(os.environ.get("QMIN_FENCE_EVERY", "4"))
+
+
+def gen_fast(tables):
+    terms = [shaped_laplacian(), compose(tables["L2"]["bulk"]), compose(tables["L3"]["bulk"]),
+             compose(tables["L4"]["bulk"])]
+    dev, host = [], ["// Generated by open-qmin_amd/stencils/gen_stencils.py.  Do not edit by hand.",
+                     "// Composed bulk operator: weight of entry k = L1 C[k][0] + L2 C[k][1] + L3 C[k][2] + L4 C[k][3].",
+                     "#pragma once"]
+    for name, used in (("a", 3), ("b", 4)):
+        keys = sorted(set().union(*[set(t) for t in terms[:used]]),
+                      key=lambda k: (POINTS.index(k[1]), k[2], k[0]))
+        host.append("static const int kFastLin%sCount = %d;" % (name.upper(), len(keys)))
+        host.append("static const double kFastLin%s[%d][4] = {" % (name.upper(), len(keys)))
+        for k in keys:
+            host.append("    {%s}," % ", ".join(repr(float(t.get(k, 0))) for t in terms))
+        host.append("};")
+        dev += ["template <typename R, typename WP>",
+                "__device__ __forceinline__ void ldg_fast_lin_%s(WP W, const R (*q)[5][kHP], int sm,"
+                " int sc, int sp, int h, R* acc) {" % name,
+                "    R v;"]
+        last = None
+        points_seen = []
+        for idx, (r, p, c) in enumerate(keys):
+            if p not in points_seen:
+                # bound the number of stencil values in flight (register pressure)
+                if points_seen and len(points_seen) % FENCE_EVERY == 0:
+                    dev.append("    __builtin_amdgcn_sched_barrier(0);")
+                points_seen.append(p)
+            if (p, c) != last:
+                slot = {-1: "sm", 0: "sc", 1: "sp"}[p[2]]
+                off = "h"
+                if p[0]:
+                    off += " %s 1" % ("+" if p[0] > 0 else "-")
+                if p[1]:
+                    off += " %s kHX" % ("+" if p[1] > 0 else "-")
+                dev.append("    v = q[%s][%d][%s];" % (slot, c, off))
+                last = (p, c)
+            dev.append("    acc[%d] += W[%d] * v;" % (r, idx))
+        dev.append("}")
+    return dev, host
+
+
 def main():
     with open(TABLES) as fh:
         doc = json.load(fh)
@@ -133,6 +238,13 @@ def main():
     gen_l6(tables["L6"]["bulk"], body)
     body += ["}", ""]
     body += gen_boundary(tables, cases)
+    fast_dev, fast_host = gen_fast(tables)
+    body += [""] + fast_dev
+    host_text = "\n".join(fast_host) + "\n"
+    host_out = os.path.join(os.path.dirname(OUT), "ldg_fast_host.inc")
+    if not os.path.exists(host_out) or open(host_out).read() != host_text:
+        with open(host_out, "w") as fh:
+            fh.write(host_text)
     os.makedirs(os.path.dirname(OUT), exist_ok=True)
     text = "\n".join(body) + "\n"
     if not os.path.exists(OUT) or open(OUT).read() != text:
