@@ -53,8 +53,9 @@ def fire_params():
                 alpha_min=0.0)
 
 
-def gpu_throughput(qmin, L, precision, steps, warmup):
+def gpu_throughput(qmin, L, precision, steps, warmup, zchunk=0):
     lat, force = build_workload(qmin, L, precision)
+    lat.set_zchunk(zchunk)
     m = qmin.FIRE(fc=1e-8, max_iter=10 ** 9, **fire_params())
     if warmup:
         m.run_iterations(force, lat, warmup)
@@ -72,6 +73,17 @@ def gpu_throughput(qmin, L, precision, steps, warmup):
     st = m.state()
     return dict(wall=wall, n=lat.n, march_ms=ms0 / max(n0, 1), boundary_ms=ms1 / max(n1, 1),
                 forceMax=st.forceMax, iterations=st.iterations, n_boundary=lat.n_boundary())
+
+
+def measured_traffic(size, precision):
+    """HBM bytes per fused-pass dispatch from the committed rocprofv3 counter
+    summary of this workload, when it covers this size and precision."""
+    path = os.path.join(REPO, "profiles", "march_kernel_fire_%s_%d.traffic.json" % (precision, size))
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        d = json.load(fh)
+    return d["read_bytes"] + d["write_bytes"]
 
 
 def cpu_baseline(L=128, iters=12):
@@ -105,6 +117,7 @@ def main():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--zchunk", type=int, default=0, help="fixed z chunk of the fused pass (0 = automatic)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build's hot path runs on a single MI355X "
@@ -113,7 +126,7 @@ def main():
     import qmin
     qmin.library()
     precision = qmin.QM_FP64 if args.precision == "fp64" else qmin.QM_FP32
-    r = gpu_throughput(qmin, args.size, precision, args.steps, args.warmup)
+    r = gpu_throughput(qmin, args.size, precision, args.steps, args.warmup, args.zchunk)
     value = r["n"] * args.steps / r["wall"]
     bps = BYTES_PER_SITE[precision]
     achieved = bps * r["n"] / (r["march_ms"] * 1e-3) / 1e9
@@ -134,7 +147,7 @@ def main():
                                % (args.size, args.size - 1, 20.0 * args.size / 256.0),
                    "sites": r["n"], "boundary_sites": r["n_boundary"], "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": measured_traffic(args.size, args.precision),
                      "kernel": "march_kernel (fused FIRE lattice pass)", "kernel_ms": r["march_ms"],
                      "boundary_kernel_ms": r["boundary_ms"], "bytes_per_site": bps},
         "iteration_hbm_frac": value * bps / (PEAK_HBM_GBS * 1e9),

@@ -56,6 +56,10 @@ int qm_device_count(int* count);
 int qm_lattice_create(int lx, int ly, int lz, int precision, qm_lattice** out);
 int qm_lattice_destroy(qm_lattice* lat);
 int qm_lattice_dims(const qm_lattice* lat, int* lx, int* ly, int* lz, int* precision);
+/* z extent each work-group of the fused lattice pass marches through (>= 4);
+ * 0 (default) sizes it so that the whole grid is resident at once.  A tuning
+ * knob of this implementation; it has no counterpart in the reference. */
+int qm_lattice_set_zchunk(qm_lattice* lat, int zchunk);
 /* AoS host <-> device state; which = QM_FIELD_Q/V/F (simpleModel::returnPositions/
  * returnVelocities/returnForces, src/model/simpleModel.h:63-73) */
 int qm_lattice_set_field(qm_lattice* lat, int which, const double* aos);

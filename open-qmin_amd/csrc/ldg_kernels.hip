@@ -572,10 +572,11 @@ __global__ void __launch_bounds__(kBoundaryThreads) boundary_kernel(BoundaryArgs
     // fixed-order sum of every partial: march blocks then boundary blocks
     const long long total = A.marchBlocks + gridDim.x;
     double s0 = 0, s1 = 0, s2 = 0;
+    const double* part = A.partials;
     for (long long j = threadIdx.x; j < total; j += kBoundaryThreads) {
-        s0 += __hip_atomic_load(&A.partials[3 * j + 0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s1 += __hip_atomic_load(&A.partials[3 * j + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s2 += __hip_atomic_load(&A.partials[3 * j + 2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s0 += part[3 * j + 0];
+        s1 += part[3 * j + 1];
+        s2 += part[3 * j + 2];
     }
     __syncthreads();
     block_sum3<kBoundaryThreads>(s0, s1, s2, sRed);
@@ -758,6 +759,20 @@ void launch_boundary(const BoundaryArgs<R>& a, int mode, int terms, hipStream_t 
     }
 }
 
+// Work-groups of the fused pass that fit on one CU at once.
+template <typename R>
+int march_blocks_per_cu(int mode, int terms) {
+    int blocks = 0;
+    const void* fn = nullptr;
+#define QMIN_PICK(M, T) if (mode == M && terms == T) fn = reinterpret_cast<const void*>(&march_kernel<R, M, T>);
+    QMIN_PICK(kFire, kOneConstant) QMIN_PICK(kFire, kMultiLinear) QMIN_PICK(kFire, kMultiL4) QMIN_PICK(kFire, kMultiL6)
+    QMIN_PICK(kForce, kOneConstant) QMIN_PICK(kForce, kMultiLinear) QMIN_PICK(kForce, kMultiL4)
+    QMIN_PICK(kForce, kMultiL6)
+#undef QMIN_PICK
+    if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fn, kThreads, 0) != hipSuccess) return 1;
+    return blocks < 1 ? 1 : blocks;
+}
+
 int boundary_blocks(long long nsites) {
     long long b = (nsites + kBoundaryThreads - 1) / kBoundaryThreads;
     return b < 1 ? 1 : int(b);
@@ -794,7 +809,8 @@ void launch_defects(const R* q, const int8_t* types, long long n, int kind, doub
     template void launch_materialize<R>(StateSet<R>, StateSet<R>, long long, Ctrl*, hipStream_t);                \
     template int launch_energy<R>(const R*, const int8_t*, const DevObject*, int, int, int, const EnergyConst&,   \
                                   double*, hipStream_t);                                                        \
-    template void launch_defects<R>(const R*, const int8_t*, long long, int, double*, hipStream_t);
+    template void launch_defects<R>(const R*, const int8_t*, long long, int, double*, hipStream_t);             \
+    template int march_blocks_per_cu<R>(int, int);
 
 QMIN_INSTANTIATE(double)
 QMIN_INSTANTIATE(float)

@@ -25,6 +25,8 @@ template <typename R>
 void launch_boundary(const BoundaryArgs<R>& a, int mode, int terms, hipStream_t s);
 int boundary_blocks(long long nsites);
 template <typename R>
+int march_blocks_per_cu(int mode, int terms);
+template <typename R>
 void launch_materialize(StateSet<R> s0, StateSet<R> s1, long long n, Ctrl* ctrl, hipStream_t s);
 template <typename R>
 int launch_energy(const R* q, const int8_t* types, const DevObject* objects, int lx, int ly, int lz,
@@ -91,7 +93,9 @@ struct qm_lattice {
     double* dTrace = nullptr;
     int traceCap = 0;
     bool typesDirty = true;
-    int zchunk = 16;
+    int zchunk = 16;          // z extent one work-group marches through
+    int zchunkOverride = 0;   // fixed z chunk (0 = choose from occupancy)
+    int numCUs = 0;
 };
 
 struct qm_ldg {
@@ -304,6 +308,13 @@ int qm_lattice_destroy(qm_lattice* L) {
     if (L->dTrace) (void)hipFree(L->dTrace);
     (void)hipStreamDestroy(L->stream);
     delete L;
+    return QM_OK;
+}
+
+int qm_lattice_set_zchunk(qm_lattice* L, int zchunk) {
+    if (!L || zchunk < 0) return fail(QM_EINVAL, "bad argument");
+    if (zchunk > 0 && zchunk < 4) return fail(QM_EINVAL, "z chunk must be 0 (automatic) or at least 4");
+    L->zchunkOverride = zchunk;
     return QM_OK;
 }
 
@@ -723,6 +734,31 @@ static void fill_args(qm_lattice* L, const qm_ldg* f, MarchArgs<R>& ma, Boundary
     ba.trace = L->dTrace;
 }
 
+// z chunk of the fused pass: as long as possible while the grid still fills
+// every CU to its occupancy, so that the whole grid is resident at once, all
+// columns march through z in step and neighbouring columns find each other's
+// halos in L2; minimum 4 planes.
+static void choose_zchunk(qm_lattice* L, int mode, int terms) {
+    if (L->zchunkOverride > 0) {
+        L->zchunk = L->zchunkOverride;
+        return;
+    }
+    if (L->numCUs == 0) {
+        int dev = 0;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+            L->numCUs = prop.multiProcessorCount;
+        if (L->numCUs <= 0) L->numCUs = 256;
+    }
+    const int perCU = L->precision == QM_FP64 ? march_blocks_per_cu<double>(mode, terms)
+                                              : march_blocks_per_cu<float>(mode, terms);
+    const long long columns = (long long)((L->lx + kTX - 1) / kTX) * ((L->ly + kTY - 1) / kTY);
+    long long chunks = (long long)perCU * L->numCUs / columns;
+    if (chunks < 1) chunks = 1;
+    int zc = (int)((L->lz + chunks - 1) / chunks);
+    L->zchunk = std::max(4, std::min(zc, L->lz));
+}
+
 static int march_blocks(const qm_lattice* L) {
     const int zc = std::min(L->zchunk, L->lz);
     return ((L->lx + kTX - 1) / kTX) * ((L->ly + kTY - 1) / kTY) * ((L->lz + zc - 1) / zc);
@@ -731,6 +767,7 @@ static int march_blocks(const qm_lattice* L) {
 // One pass (force only or one FIRE iteration); kernels only, no host sync.
 template <typename R>
 static int run_pass(qm_lattice* L, const qm_ldg* f, int mode, const FireConst& fire) {
+    choose_zchunk(L, mode, terms_of(f));
     MarchArgs<R> ma{};
     BoundaryArgs<R> ba{};
     fill_args<R>(L, f, ma, ba, fire);
@@ -764,7 +801,9 @@ static int prepare(qm_lattice* L, const qm_ldg* f) {
     if (!L || !f) return fail(QM_EINVAL, "null handle");
     if (int rc = refresh_types(L)) return rc;
     if (int rc = upload_weights(L, f)) return rc;
-    const size_t need = 3 * (size_t)(march_blocks(L) + boundary_blocks(L->nBoundary));
+    // room for the smallest z chunk (4 planes) the fused pass may pick
+    const long long worst = (long long)((L->lx + kTX - 1) / kTX) * ((L->ly + kTY - 1) / kTY) * ((L->lz + 3) / 4);
+    const size_t need = 3 * (size_t)(worst + boundary_blocks(L->nBoundary));
     return ensure_partials(L, need);
 }
 

@@ -41,6 +41,7 @@ _API = {
     "qm_lattice_create": [_i, _i, _i, _i, ctypes.POINTER(_VP)],
     "qm_lattice_destroy": [_VP],
     "qm_lattice_dims": [_VP] + [ctypes.POINTER(ctypes.c_int)] * 4,
+    "qm_lattice_set_zchunk": [_VP, _i],
     "qm_lattice_set_field": [_VP, _i, _D],
     "qm_lattice_get_field": [_VP, _i, _D],
     "qm_lattice_random_init": [_VP, _i, _d],
@@ -152,6 +153,9 @@ class Lattice:
 
     def __del__(self):
         self.close()
+
+    def set_zchunk(self, zchunk):
+        _check(self.lib.qm_lattice_set_zchunk(self.h, zchunk))
 
     def set_field(self, which, arr):
         a = np.ascontiguousarray(arr, np.float64).reshape(self.n, 5)
