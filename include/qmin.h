@@ -185,6 +185,16 @@ int qm_lolbfgs_set_max_iterations(qm_lolbfgs* m, int maxIterations);
 int qm_lolbfgs_minimize(qm_lolbfgs* m, qm_ldg* f, qm_lattice* lat);
 int qm_lolbfgs_get_state(const qm_lolbfgs* m, int* iterations, double* forceMax);
 
+/* per-iteration forceMax traces of the three minimizers above (the quantity their
+ * minimize() loops print every 1000 steps): capacity in iterations (0 disables);
+ * get copies up to cap values and returns the count in *n */
+int qm_adam_set_trace(qm_adam* m, int capacity);
+int qm_adam_get_trace(const qm_adam* m, double* forceMax, int cap, int* n);
+int qm_nesterov_set_trace(qm_nesterov* m, int capacity);
+int qm_nesterov_get_trace(const qm_nesterov* m, double* forceMax, int cap, int* n);
+int qm_lolbfgs_set_trace(qm_lolbfgs* m, int capacity);
+int qm_lolbfgs_get_trace(const qm_lolbfgs* m, double* forceMax, int cap, int* n);
+
 /* ---- kernel timing (HIP events on the library's stream) -------------------
  * When enabled, every launch of the two FIRE kernels is bracketed by events;
  * qm_timing_get reports launches and total milliseconds per kernel:

@@ -170,7 +170,7 @@ __global__ void __launch_bounds__(kThreads) march_kernel(MarchArgs<R> A) {
     __shared__ double sRed[kThreads / 64][3];
 
     Ctrl* ctrl = A.ctrl;
-    if (MODE == kFire && ctrl->done) return;
+    if ((MODE == kFire || A.gate) && ctrl->done) return;
     const int cur = ctrl->cur;
     const StateSet<R> in = A.set[cur];
     const StateSet<R> out = A.set[MODE == kFire ? cur ^ 1 : cur];
@@ -407,7 +407,7 @@ __global__ void __launch_bounds__(kBoundaryThreads) boundary_kernel(BoundaryArgs
     __shared__ double sRed[kBoundaryThreads / 64][3];
     __shared__ int sLast;
     Ctrl* ctrl = A.ctrl;
-    if (MODE == kFire && ctrl->done) return;
+    if ((MODE == kFire || A.gate) && ctrl->done) return;
     const int cur = ctrl->cur;
     const StateSet<R> st = A.set[MODE == kFire ? cur ^ 1 : cur];
     const R hdt = MODE == kFire ? R(0.5 * ctrl->dt) : R(0);

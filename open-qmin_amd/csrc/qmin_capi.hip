@@ -13,8 +13,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/qmin.h"
-#include "qmin_device.h"
+#include "qmin_internal.h"
 #include "host_init.h"
 #include "gen/ldg_fast_host.inc"
 
@@ -33,10 +32,6 @@ int launch_energy(const R* q, const int8_t* types, const DevObject* objects, int
                   const EnergyConst& ec, double* partials, hipStream_t s);
 template <typename R>
 void launch_defects(const R* q, const int8_t* types, long long n, int kind, double* out, hipStream_t s);
-// minimizer passes (minimizers.hip)
-int adam_minimize_impl(qm_adam* m, qm_ldg* f, qm_lattice* lat);
-int nesterov_minimize_impl(qm_nesterov* m, qm_ldg* f, qm_lattice* lat);
-int lolbfgs_minimize_impl(qm_lolbfgs* m, qm_ldg* f, qm_lattice* lat);
 }  // namespace qmin
 
 using namespace qmin;
@@ -44,17 +39,10 @@ using namespace qmin;
 // ------------------------------------------------------------------ errors
 static thread_local std::string g_error;
 
-static int fail(int code, const std::string& msg) {
+int qmin::fail(int code, const std::string& msg) {
     g_error = msg;
     return code;
 }
-
-#define HIP_TRY(expr)                                                                       \
-    do {                                                                                    \
-        hipError_t e_ = (expr);                                                             \
-        if (e_ != hipSuccess)                                                               \
-            return fail(QM_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));       \
-    } while (0)
 
 // ------------------------------------------------------------------ timing
 namespace {
@@ -65,61 +53,6 @@ struct Timing {
 };
 Timing g_timing;
 }  // namespace
-
-// ------------------------------------------------------------------ handles
-struct HostObject {
-    int type;
-    double W, S0;
-};
-
-struct qm_lattice {
-    int lx = 0, ly = 0, lz = 0, precision = QM_FP64;
-    long long n = 0;
-    size_t realSize = 8;
-    hipStream_t stream = nullptr;
-    std::vector<int32_t> types;      // host copy, reference encoding
-    std::vector<HostObject> objects;
-    int8_t* dTypes = nullptr;
-    int8_t* dClass = nullptr;             // 0 composed bulk formula, > 0 object, < 0 boundary pass
-    void* dW = nullptr;                   // composed bulk operator weights (Real)
-    DevObject* dObjects = nullptr;
-    void* dSet[2] = {nullptr, nullptr};   // each 15 n reals: Q, U, F planes
-    int32_t* dSites = nullptr;            // compacted sites finished by the boundary pass
-    long long nBoundary = 0;
-    Ctrl* dCtrl = nullptr;
-    Ctrl hCtrl{};                          // host mirror, valid after syncCtrl
-    double* dPartials = nullptr;
-    size_t partialsCap = 0;
-    double* dTrace = nullptr;
-    int traceCap = 0;
-    bool typesDirty = true;
-    int zchunk = 16;          // z extent one work-group marches through
-    int zchunkOverride = 0;   // fixed z chunk (0 = choose from occupancy)
-    int numCUs = 0;
-};
-
-struct qm_ldg {
-    double A = -1, B = 0, C = 0;
-    double L1 = 2.32, L2 = 0, L3 = 0, L4 = 0, L6 = 0;
-    int multi = 0;
-    int semantics = QM_SEMANTICS_REF_GPU;
-    int eOn = 0, hOn = 0;
-    double E[3] = {0, 0, 0}, eps = 1, eps0 = 1, deps = 0.5;
-    double H[3] = {0, 0, 0}, chi = 1, mu0 = 1, dchi = 0.5;
-};
-
-struct qm_fire {
-    // energyMinimizerFIRE::initializeParameters defaults (energyMinimizerFIRE.cpp:24-43)
-    double dt = 0.01, dtMin = 0.0001, alphaStart = 0.99, alpha = 0.99, dtMax = 0.1, dtInc = 1.05, dtDec = 0.95,
-           alphaDec = 0.9, alphaMin = 0.0, fc = 1e-7;
-    int nMin = 5, maxIter = 1000;
-    int iterations = 0, nPositive = 0;
-    double forceMax = 100.0, power = 0, scaling = 0;
-    int converged = 0, nan = 0;
-    int traceCap = 0;
-    std::vector<double> trace;
-    qm_lattice* bound = nullptr;   // lattice holding this minimizer's device scalars
-};
 
 const char* qm_last_error(void) { return g_error.c_str(); }
 
@@ -144,19 +77,13 @@ static int wrapc(int v, int m) {
     return a;
 }
 
-template <typename R>
-static StateSet<R> state_set(const qm_lattice* L, int s) {
-    R* base = static_cast<R*>(L->dSet[s]);
-    return StateSet<R>{base, base + 5 * L->n, base + 10 * L->n};
-}
-
-static int sync_ctrl_to_host(qm_lattice* L) {
+int qmin::sync_ctrl_to_host(qm_lattice* L) {
     HIP_TRY(hipMemcpyAsync(&L->hCtrl, L->dCtrl, sizeof(Ctrl), hipMemcpyDeviceToHost, L->stream));
     HIP_TRY(hipStreamSynchronize(L->stream));
     return QM_OK;
 }
 
-static int push_ctrl(qm_lattice* L) {
+int qmin::push_ctrl(qm_lattice* L) {
     HIP_TRY(hipMemcpyAsync(L->dCtrl, &L->hCtrl, sizeof(Ctrl), hipMemcpyHostToDevice, L->stream));
     HIP_TRY(hipStreamSynchronize(L->stream));
     return QM_OK;
@@ -214,7 +141,7 @@ static int refresh_types(qm_lattice* L) {
     return QM_OK;
 }
 
-static int ensure_partials(qm_lattice* L, size_t count) {
+int qmin::ensure_partials(qm_lattice* L, size_t count) {
     if (count <= L->partialsCap) return QM_OK;
     if (L->dPartials) HIP_TRY(hipFree(L->dPartials));
     HIP_TRY(hipMalloc(&L->dPartials, count * sizeof(double)));
@@ -338,7 +265,6 @@ int qm_lattice_set_field(qm_lattice* L, int which, const double* aos) {
     return upload_field(L, which, aos);
 }
 
-static int materialize(qm_lattice* L);
 
 int qm_lattice_get_field(qm_lattice* L, int which, double* aos) {
     if (!L || !aos || which < 0 || which > 2) return fail(QM_EINVAL, "bad argument");
@@ -766,11 +692,13 @@ static int march_blocks(const qm_lattice* L) {
 
 // One pass (force only or one FIRE iteration); kernels only, no host sync.
 template <typename R>
-static int run_pass(qm_lattice* L, const qm_ldg* f, int mode, const FireConst& fire) {
+static int run_pass(qm_lattice* L, const qm_ldg* f, int mode, const FireConst& fire, bool gated = false) {
     choose_zchunk(L, mode, terms_of(f));
     MarchArgs<R> ma{};
     BoundaryArgs<R> ba{};
     fill_args<R>(L, f, ma, ba, fire);
+    ma.gate = gated ? 1 : 0;
+    ba.gate = ma.gate;
     const int terms = terms_of(f);
     hipEvent_t ev[3];
     if (g_timing.on && mode == kFire) {
@@ -797,7 +725,7 @@ static int run_pass(qm_lattice* L, const qm_ldg* f, int mode, const FireConst& f
     return QM_OK;
 }
 
-static int prepare(qm_lattice* L, const qm_ldg* f) {
+int qmin::prepare(qm_lattice* L, const qm_ldg* f) {
     if (!L || !f) return fail(QM_EINVAL, "null handle");
     if (int rc = refresh_types(L)) return rc;
     if (int rc = upload_weights(L, f)) return rc;
@@ -805,6 +733,18 @@ static int prepare(qm_lattice* L, const qm_ldg* f) {
     const long long worst = (long long)((L->lx + kTX - 1) / kTX) * ((L->ly + kTY - 1) / kTY) * ((L->lz + 3) / 4);
     const size_t need = 3 * (size_t)(worst + boundary_blocks(L->nBoundary));
     return ensure_partials(L, need);
+}
+
+int qmin::force_pass(qm_lattice* L, const qm_ldg* f, bool gated) {
+    FireConst fire{};
+    return L->precision == QM_FP64 ? run_pass<double>(L, f, kForce, fire, gated)
+                                   : run_pass<float>(L, f, kForce, fire, gated);
+}
+
+int64_t qmin::n_active(const qm_lattice* L) {
+    int64_t c = 0;
+    for (long long i = 0; i < L->n; ++i) c += L->types[i] <= 0;
+    return c;
 }
 
 int qm_ldg_compute_forces(qm_ldg* f, qm_lattice* L) {
@@ -861,7 +801,7 @@ int qm_ldg_compute_energy(qm_ldg* f, qm_lattice* L, double* total, double* comps
 }
 
 // ------------------------------------------------------------------ FIRE
-static int materialize(qm_lattice* L) {
+int qmin::materialize(qm_lattice* L) {
     if (int rc = sync_ctrl_to_host(L)) return rc;
     if (L->hCtrl.identity) return QM_OK;
     if (L->precision == QM_FP64)
@@ -1042,5 +982,3 @@ int qm_timing_get(int k, int64_t* launches, double* ms) {
     return QM_OK;
 }
 
-// ------------------------------------------------------------------ Adam / Nesterov / LoLBFGS handles
-// (passes in minimizers.hip)

@@ -55,6 +55,7 @@ struct Ctrl {
     int pad;
     double dt, alpha;
     double cKeep, cForce;   // pending mix v = cKeep u + cForce F
+    double slot[40];        // minimizer scalars finished on the device (L-BFGS s.y, a_i, coefficients)
     double forceMax, power, fnorm, vnorm, scaling;
 };
 
@@ -83,6 +84,7 @@ template <typename R>
 struct MarchArgs {
     StateSet<R> set[2];
     const int8_t* cls;      // per site: 0 composed bulk formula, > 0 object, < 0 finished by the boundary pass
+    int gate;               // force-only pass: skip once the control block reports done
     int lx, ly, lz, zchunk;
     long long n;
     int nbx, nby, nblocks;
@@ -97,7 +99,8 @@ struct BoundaryArgs {
     StateSet<R> set[2];
     const int8_t* types;
     const DevObject* objects;
-    const int32_t* sites;   // compacted list of t < 0 sites
+    const int32_t* sites;   // compacted list of sites finished here
+    int gate;
     long long nsites;
     int lx, ly, lz;
     long long n;

@@ -1,0 +1,95 @@
+// Internal definitions shared by the host translation units of libqmin.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/qmin.h"
+#include "qmin_device.h"
+
+using qmin::Ctrl;
+using qmin::DevObject;
+using qmin::StateSet;
+
+struct HostObject {
+    int type;
+    double W, S0;
+};
+
+struct qm_lattice {
+    int lx = 0, ly = 0, lz = 0, precision = QM_FP64;
+    long long n = 0;
+    size_t realSize = 8;
+    hipStream_t stream = nullptr;
+    std::vector<int32_t> types;      // host copy, reference encoding
+    std::vector<HostObject> objects;
+    int8_t* dTypes = nullptr;
+    int8_t* dClass = nullptr;             // 0 composed bulk formula, > 0 object, < 0 boundary pass
+    void* dW = nullptr;                   // composed bulk operator weights (Real)
+    DevObject* dObjects = nullptr;
+    void* dSet[2] = {nullptr, nullptr};   // each 15 n reals: Q, U, F planes
+    int32_t* dSites = nullptr;            // compacted sites finished by the boundary pass
+    long long nBoundary = 0;
+    Ctrl* dCtrl = nullptr;
+    Ctrl hCtrl{};                          // host mirror, valid after syncCtrl
+    double* dPartials = nullptr;
+    size_t partialsCap = 0;
+    double* dTrace = nullptr;
+    int traceCap = 0;
+    bool typesDirty = true;
+    int zchunk = 16;          // z extent one work-group marches through
+    int zchunkOverride = 0;   // fixed z chunk (0 = choose from occupancy)
+    int numCUs = 0;
+};
+
+struct qm_ldg {
+    double A = -1, B = 0, C = 0;
+    double L1 = 2.32, L2 = 0, L3 = 0, L4 = 0, L6 = 0;
+    int multi = 0;
+    int semantics = QM_SEMANTICS_REF_GPU;
+    int eOn = 0, hOn = 0;
+    double E[3] = {0, 0, 0}, eps = 1, eps0 = 1, deps = 0.5;
+    double H[3] = {0, 0, 0}, chi = 1, mu0 = 1, dchi = 0.5;
+};
+
+struct qm_fire {
+    // energyMinimizerFIRE::initializeParameters defaults (energyMinimizerFIRE.cpp:24-43)
+    double dt = 0.01, dtMin = 0.0001, alphaStart = 0.99, alpha = 0.99, dtMax = 0.1, dtInc = 1.05, dtDec = 0.95,
+           alphaDec = 0.9, alphaMin = 0.0, fc = 1e-7;
+    int nMin = 5, maxIter = 1000;
+    int iterations = 0, nPositive = 0;
+    double forceMax = 100.0, power = 0, scaling = 0;
+    int converged = 0, nan = 0;
+    int traceCap = 0;
+    std::vector<double> trace;
+    qm_lattice* bound = nullptr;   // lattice holding this minimizer's device scalars
+};
+
+
+namespace qmin {
+int fail(int code, const std::string& msg);
+// type upload, partial-sum storage and weights before a pass
+int prepare(qm_lattice* L, const qm_ldg* f);
+// F = forces(Q) into the current state set; gated passes are skipped on the
+// device once the control block reports done
+int force_pass(qm_lattice* L, const qm_ldg* f, bool gated);
+int sync_ctrl_to_host(qm_lattice* L);
+// makes a pending FIRE velocity mix concrete in U of the current state set
+int materialize(qm_lattice* L);
+int push_ctrl(qm_lattice* L);
+int ensure_partials(qm_lattice* L, size_t count);
+int64_t n_active(const qm_lattice* L);
+template <typename R>
+inline StateSet<R> state_set(const qm_lattice* L, int s) {
+    R* base = static_cast<R*>(L->dSet[s]);
+    return StateSet<R>{base, base + 5 * L->n, base + 10 * L->n};
+}
+}  // namespace qmin
+
+#define HIP_TRY(expr)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return qmin::fail(QM_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)

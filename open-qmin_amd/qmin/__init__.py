@@ -82,6 +82,12 @@ _API = {
     "qm_adam_set_max_iterations": [_VP, _i],
     "qm_adam_minimize": [_VP, _VP, _VP],
     "qm_adam_get_state": [_VP, ctypes.POINTER(ctypes.c_int), _D],
+    "qm_adam_set_trace": [_VP, _i],
+    "qm_adam_get_trace": [_VP, _D, _i, ctypes.POINTER(ctypes.c_int)],
+    "qm_nesterov_set_trace": [_VP, _i],
+    "qm_nesterov_get_trace": [_VP, _D, _i, ctypes.POINTER(ctypes.c_int)],
+    "qm_lolbfgs_set_trace": [_VP, _i],
+    "qm_lolbfgs_get_trace": [_VP, _D, _i, ctypes.POINTER(ctypes.c_int)],
     "qm_nesterov_create": [ctypes.POINTER(_VP)],
     "qm_nesterov_destroy": [_VP],
     "qm_nesterov_set_parameters": [_VP, _d, _d, _d, _i],
@@ -366,6 +372,19 @@ class _SimpleMinimizer:
         fm = ctypes.c_double()
         _check(getattr(self.lib, self._prefix + "_get_state")(self.h, ctypes.byref(it), ctypes.byref(fm)))
         return it.value, fm.value
+
+    def set_trace(self, capacity):
+        """Record forceMax of every iteration (up to capacity)."""
+        self._trace_cap = capacity
+        _check(getattr(self.lib, self._prefix + "_set_trace")(self.h, capacity))
+
+    def trace(self):
+        cap = getattr(self, "_trace_cap", 0)
+        out = np.zeros(max(cap, 1))
+        n = ctypes.c_int()
+        _check(getattr(self.lib, self._prefix + "_get_trace")(
+            self.h, out.ctypes.data_as(_D), cap, ctypes.byref(n)))
+        return out[:n.value].copy()
 
 
 class Adam(_SimpleMinimizer):

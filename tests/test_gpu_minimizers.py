@@ -1,0 +1,116 @@
+"""Adam, Nesterov AG and online L-BFGS on the MI355X against the CPU oracle.
+
+The oracle's minimizers reproduce the reference's traces bit for bit
+(test_oracle_reference.py); here the device runs the same iterations from the
+same seeded state.  Tolerances: per-iteration forceMax within 1e-10 relative and
+final Q within 1e-11 absolute in FP64 (the device sums its reductions in a tree
+order, the oracle serially); FP32 state within 1e-4 relative on forceMax.
+"""
+import numpy as np
+import pytest
+
+import qmin
+from test_gpu_parity import ALL, MULTI, ONE, build_pair
+
+pytestmark = pytest.mark.gpu
+
+
+def check(tr, tr_ref, g, o, rtol=1e-10, atol_q=1e-11):
+    assert tr.shape == tr_ref.shape, (tr.shape, tr_ref.shape)
+    np.testing.assert_allclose(tr, tr_ref, rtol=rtol)
+    np.testing.assert_allclose(g.q(), o.q(), rtol=0, atol=atol_q)
+
+
+@pytest.mark.parametrize("params", [ONE, MULTI, ALL])
+def test_adam_trajectory(params):
+    o, g, f = build_pair((20, 18, 12), params, 0, True)
+    it_ref, tr_ref = o.adam(dt=1e-3, max_iter=40)
+    m = qmin.Adam(dt=1e-3, max_iter=40)
+    m.set_trace(64)
+    it, fm = m.minimize(f, g)
+    assert it == it_ref == 40
+    check(m.trace(), tr_ref, g, o)
+    assert fm == pytest.approx(tr_ref[-1], rel=1e-10)
+
+
+def test_adam_stops_on_tolerance():
+    o, g, f = build_pair((16, 12, 10), ONE, 1, True)
+    it_ref, tr_ref = o.adam(dt=2e-3, fc=0.05, max_iter=2000)
+    m = qmin.Adam(dt=2e-3, fc=0.05, max_iter=2000)
+    it, fm = m.minimize(f, g)
+    assert it == it_ref < 2000
+    assert fm <= 0.05
+    np.testing.assert_allclose(g.q(), o.q(), rtol=0, atol=1e-10)
+
+
+@pytest.mark.parametrize("scheduled,mu,dt", [(False, 0.5, 1e-2), (True, 0.5, 1e-2), (False, 0.01, 5e-3)])
+def test_nesterov_trajectory(scheduled, mu, dt):
+    o, g, f = build_pair((22, 14, 12), MULTI, 0, True)
+    it_ref, tr_ref = o.nesterov(dt=dt, mu=mu, scheduled=scheduled, max_iter=40)
+    m = qmin.Nesterov(dt=dt, mu=mu, scheduled=scheduled, max_iter=40)
+    m.set_trace(64)
+    it, _ = m.minimize(f, g)
+    assert it == it_ref == 40
+    check(m.trace(), tr_ref, g, o)
+
+
+@pytest.mark.parametrize("mem,scheduled", [(5, False), (8, False), (5, True)])
+def test_lolbfgs_trajectory(mem, scheduled):
+    o, g, f = build_pair((20, 16, 14), MULTI, 0, True)
+    it_ref, tr_ref = o.lolbfgs(m=mem, dt=1e-2, scheduled=scheduled, max_iter=30)
+    m = qmin.LoLBFGS(m=mem, dt=1e-2, scheduled=scheduled, max_iter=30)
+    m.set_trace(64)
+    it, _ = m.minimize(f, g)
+    assert it == it_ref == 30
+    check(m.trace(), tr_ref, g, o, rtol=1e-9, atol_q=1e-10)
+
+
+def test_lolbfgs_all_constants_periodic():
+    o, g, f = build_pair((18, 18, 18), ALL, 1, False)
+    it_ref, tr_ref = o.lolbfgs(m=5, dt=1e-2, max_iter=25)
+    m = qmin.LoLBFGS(m=5, dt=1e-2, max_iter=25)
+    m.set_trace(32)
+    m.minimize(f, g)
+    check(m.trace(), tr_ref, g, o, rtol=1e-9, atol_q=1e-10)
+
+
+def test_lolbfgs_continues_across_calls():
+    """Two minimize calls of 12 + 18 iterations equal one of 30: the history,
+    the direction and the slot counter persist on the device."""
+    o, g, f = build_pair((18, 14, 12), MULTI, 0, True)
+    it_ref, tr_ref = o.lolbfgs(m=5, dt=1e-2, max_iter=30)
+    m = qmin.LoLBFGS(m=5, dt=1e-2, max_iter=12)
+    m.set_trace(64)
+    m.minimize(f, g)
+    m.set_max_iterations(30)
+    it, _ = m.minimize(f, g)
+    assert it == 30
+    check(m.trace(), tr_ref, g, o, rtol=1e-9, atol_q=1e-10)
+
+
+def test_nesterov_then_fire_keeps_state_consistent():
+    """Switching minimizers on one lattice: FIRE after Nesterov starts from the
+    Nesterov result, as with the reference's shared model."""
+    o, g, f = build_pair((16, 14, 12), MULTI, 0, True)
+    o.nesterov(dt=1e-2, mu=0.5, max_iter=10)
+    o.fire(max_iter=15)
+    qmin.Nesterov(dt=1e-2, mu=0.5, max_iter=10).minimize(f, g)
+    qmin.FIRE(max_iter=15).minimize(f, g)
+    np.testing.assert_allclose(g.q(), o.q(), rtol=0, atol=1e-11)
+
+
+def test_fp32_minimizers_track_fp64():
+    for name in ("adam", "nesterov", "lolbfgs"):
+        o, g, f = build_pair((20, 16, 12), MULTI, 1, True, precision=qmin.QM_FP32)
+        if name == "adam":
+            _, tr_ref = o.adam(dt=1e-3, max_iter=20)
+            m = qmin.Adam(dt=1e-3, max_iter=20)
+        elif name == "nesterov":
+            _, tr_ref = o.nesterov(dt=1e-2, mu=0.5, max_iter=20)
+            m = qmin.Nesterov(dt=1e-2, mu=0.5, max_iter=20)
+        else:
+            _, tr_ref = o.lolbfgs(m=5, dt=1e-2, max_iter=20)
+            m = qmin.LoLBFGS(m=5, dt=1e-2, max_iter=20)
+        m.set_trace(32)
+        m.minimize(f, g)
+        np.testing.assert_allclose(m.trace(), tr_ref, rtol=1e-4, err_msg=name)
