@@ -60,6 +60,11 @@ int qm_lattice_dims(const qm_lattice* lat, int* lx, int* ly, int* lz, int* preci
  * 0 (default) sizes it so that the whole grid is resident at once.  A tuning
  * knob of this implementation; it has no counterpart in the reference. */
 int qm_lattice_set_zchunk(qm_lattice* lat, int zchunk);
+/* converts the device state (Q, v, F) to the other storage precision, e.g. for an
+ * FP64 polish after an FP32 minimization; minimizers bound to the lattice other
+ * than FIRE restart their history.  No reference counterpart (the reference is
+ * FP64 throughout). */
+int qm_lattice_set_precision(qm_lattice* lat, int precision);
 /* AoS host <-> device state; which = QM_FIELD_Q/V/F (simpleModel::returnPositions/
  * returnVelocities/returnForces, src/model/simpleModel.h:63-73) */
 int qm_lattice_set_field(qm_lattice* lat, int which, const double* aos);

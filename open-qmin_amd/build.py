@@ -9,6 +9,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "build", "obj")
 LIB = os.path.join(HERE, "lib", "libqmin.so")
+DRIVER = os.path.join(HERE, "lib", "openQmin-hip")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 SOURCES = ["ldg_kernels.hip", "qmin_capi.hip", "minimizers.hip"]
 # host code that must match the reference's g++ build bit for bit
 HOST_SOURCES = ["host_init.cpp"]
@@ -50,6 +52,11 @@ def build(verbose=False):
     tmp = LIB + ".tmp"
     run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    # the openQmin-style C++ driver over the drop-in classes
+    run([GXX, "-std=c++17", "-O2", "-Wall", "-I" + INCLUDE, "-I" + os.path.join(HERE, "host"),
+         os.path.join(HERE, "host", "openqmin_hip.cpp"), "-o", DRIVER + ".tmp", "-L" + os.path.dirname(LIB),
+         "-lqmin", "-Wl,-rpath,$ORIGIN"])
+    os.replace(DRIVER + ".tmp", DRIVER)
     return LIB
 
 

@@ -238,6 +238,47 @@ int qm_lattice_destroy(qm_lattice* L) {
     return QM_OK;
 }
 
+template <typename S, typename D>
+__global__ void convert_kernel(const S* src, D* dst, long long count) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (long long)gridDim.x * blockDim.x)
+        dst[i] = D(src[i]);
+}
+
+// Converts the device state (Q, velocity with any pending FIRE mix applied,
+// forces) to the other storage precision in place of the old buffers.
+int qm_lattice_set_precision(qm_lattice* L, int precision) {
+    if (!L) return fail(QM_EINVAL, "null lattice");
+    if (precision != QM_FP64 && precision != QM_FP32) return fail(QM_EINVAL, "precision must be QM_FP64 or QM_FP32");
+    if (precision == L->precision) return QM_OK;
+    if (int rc = materialize(L)) return rc;
+    const size_t realSize = precision == QM_FP64 ? 8 : 4;
+    const long long count = 15 * L->n;
+    const int blocks = (int)std::min<long long>((count + 255) / 256, 8192);
+    void* fresh[2] = {nullptr, nullptr};
+    for (int s = 0; s < 2; ++s) {
+        if (hipMalloc(&fresh[s], count * realSize) != hipSuccess) {
+            if (fresh[0]) (void)hipFree(fresh[0]);
+            return fail(QM_ENOMEM, "precision change: out of device memory");
+        }
+        if (precision == QM_FP64)
+            convert_kernel<float, double><<<blocks, 256, 0, L->stream>>>(static_cast<const float*>(L->dSet[s]),
+                                                                        static_cast<double*>(fresh[s]), count);
+        else
+            convert_kernel<double, float><<<blocks, 256, 0, L->stream>>>(static_cast<const double*>(L->dSet[s]),
+                                                                        static_cast<float*>(fresh[s]), count);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipStreamSynchronize(L->stream));
+    for (int s = 0; s < 2; ++s) {
+        HIP_TRY(hipFree(L->dSet[s]));
+        L->dSet[s] = fresh[s];
+    }
+    L->precision = precision;
+    L->realSize = realSize;
+    return QM_OK;
+}
+
 int qm_lattice_set_zchunk(qm_lattice* L, int zchunk) {
     if (!L || zchunk < 0) return fail(QM_EINVAL, "bad argument");
     if (zchunk > 0 && zchunk < 4) return fail(QM_EINVAL, "z chunk must be 0 (automatic) or at least 4");

@@ -42,6 +42,7 @@ _API = {
     "qm_lattice_destroy": [_VP],
     "qm_lattice_dims": [_VP] + [ctypes.POINTER(ctypes.c_int)] * 4,
     "qm_lattice_set_zchunk": [_VP, _i],
+    "qm_lattice_set_precision": [_VP, _i],
     "qm_lattice_set_field": [_VP, _i, _D],
     "qm_lattice_get_field": [_VP, _i, _D],
     "qm_lattice_random_init": [_VP, _i, _d],
@@ -162,6 +163,11 @@ class Lattice:
 
     def set_zchunk(self, zchunk):
         _check(self.lib.qm_lattice_set_zchunk(self.h, zchunk))
+
+    def set_precision(self, precision):
+        """Convert the device state to QM_FP64 or QM_FP32 (e.g. an FP64 polish)."""
+        _check(self.lib.qm_lattice_set_precision(self.h, precision))
+        self.precision = precision
 
     def set_field(self, which, arr):
         a = np.ascontiguousarray(arr, np.float64).reshape(self.n, 5)

@@ -158,3 +158,20 @@ def test_fire_continues_across_calls():
     m.minimize(f, g)
     assert m.state().iterations == 40
     np.testing.assert_allclose(g.q(), o.q(), rtol=0, atol=1e-11)
+
+
+def test_precision_conversion_round_trip():
+    """FP64 -> FP32 rounds Q, v, F to float; FP32 -> FP64 is exact; FIRE continues."""
+    o, g, f = build_pair((14, 12, 10), MULTI, 1, True)
+    m = qmin.FIRE(max_iter=5)
+    m.minimize(f, g)
+    q64, v64, f64 = g.q(), g.v(), g.f()
+    g.set_precision(qmin.QM_FP32)
+    assert np.array_equal(g.q(), q64.astype(np.float32).astype(np.float64))
+    assert np.array_equal(g.v(), v64.astype(np.float32).astype(np.float64))
+    assert np.array_equal(g.f(), f64.astype(np.float32).astype(np.float64))
+    g.set_precision(qmin.QM_FP64)
+    assert np.array_equal(g.q(), q64.astype(np.float32).astype(np.float64))
+    m.set_max_iterations(10)
+    st = m.minimize(f, g)
+    assert st.iterations == 10 and np.isfinite(st.forceMax)
