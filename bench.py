@@ -75,6 +75,34 @@ def gpu_throughput(qmin, L, precision, steps, warmup, zchunk=0):
                 forceMax=st.forceMax, iterations=st.iterations, n_boundary=lat.n_boundary())
 
 
+def time_to_tolerance(qmin, L, mode, fc=1e-8, max_iter=200000):
+    """FIRE from the seeded random start until forceMax < fc (the reference's
+    definition, energyMinimizerFIRE.cpp:110).  mode "fp64": all FP64; "mixed":
+    FP32 state and stencils with FP64 reductions to fc, then the state converted
+    on the device and an FP64 polish FIRE run to the same fc."""
+    prec = qmin.QM_FP64 if mode == "fp64" else qmin.QM_FP32
+    lat, force = build_workload(qmin, L, prec)
+    m = qmin.FIRE(fc=fc, max_iter=max_iter, **fire_params())
+    t0 = time.perf_counter()
+    st = m.minimize(force, lat)
+    first = st.iterations
+    t1 = time.perf_counter()
+    if mode == "mixed":
+        lat.set_precision(qmin.QM_FP64)
+        m.set_max_iterations(first + max_iter)
+        st = m.minimize(force, lat)
+    t2 = time.perf_counter()
+    e, _ = force.compute_energy(lat)
+    out = dict(seconds=t2 - t0, iterations=st.iterations, forceMax=st.forceMax, converged=bool(st.converged),
+               energy_per_site=e / lat.n_active())
+    if mode == "mixed":
+        out.update(fp32_iterations=first, fp32_seconds=t1 - t0, polish_iterations=st.iterations - first,
+                   polish_seconds=t2 - t1)
+    print("time to forceMax<%g (%s): %d iterations, %.2f s" % (fc, mode, st.iterations, t2 - t0), file=sys.stderr,
+          flush=True)
+    return out
+
+
 def measured_traffic(size, precision):
     """HBM bytes per fused-pass dispatch from the committed rocprofv3 counter
     summary of this workload, when it covers this size and precision."""
@@ -118,6 +146,8 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--zchunk", type=int, default=0, help="fixed z chunk of the fused pass (0 = automatic)")
+    ap.add_argument("--no-ttt", action="store_true", help="skip the time-to-tolerance runs")
+    ap.add_argument("--ttt-fc", type=float, default=1e-8, help="forceMax tolerance of the time-to-tolerance runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build's hot path runs on a single MI355X "
@@ -153,6 +183,15 @@ def main():
         "iteration_hbm_frac": value * bps / (PEAK_HBM_GBS * 1e9),
         "forceMax_after": r["forceMax"],
     }
+    if not args.no_ttt:
+        t64 = time_to_tolerance(qmin, args.size, "fp64", args.ttt_fc)
+        tmx = time_to_tolerance(qmin, args.size, "mixed", args.ttt_fc)
+        out["time_to_tolerance"] = {
+            "forceMax_below": args.ttt_fc, "fp64": t64, "mixed_fp32_then_fp64_polish": tmx,
+            "mixed_speedup": t64["seconds"] / tmx["seconds"],
+            "energy_rel_diff": abs(tmx["energy_per_site"] - t64["energy_per_site"]) / abs(t64["energy_per_site"]),
+            "warm_start": None,
+        }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

@@ -60,6 +60,9 @@ int qm_lattice_dims(const qm_lattice* lat, int* lx, int* ly, int* lz, int* preci
  * 0 (default) sizes it so that the whole grid is resident at once.  A tuning
  * knob of this implementation; it has no counterpart in the reference. */
 int qm_lattice_set_zchunk(qm_lattice* lat, int zchunk);
+/* tile shape of the fused lattice pass: -1 (default) picks by precision, 0 is
+ * 32x8 sites per work-group, 1 is 64x4.  A tuning knob of this implementation. */
+int qm_lattice_set_tile(qm_lattice* lat, int tile);
 /* converts the device state (Q, v, F) to the other storage precision, e.g. for an
  * FP64 polish after an FP32 minimization; minimizers bound to the lattice other
  * than FIRE restart their history.  No reference counterpart (the reference is

@@ -15,9 +15,10 @@
 // kernel's register budget.
 //
 // Boundary pass (boundary_kernel): one thread per site of the compacted t < 0
-// list; 27-case stencils, anchoring, fields and metric; then the last
-// work-group to arrive sums all partials in a fixed order and advances the FIRE
-// scalars on the device (energyMinimizerFIRE.cpp:77-145): no host round trip.
+// list; 27-case stencils, anchoring, fields and metric.  A one-work-group
+// kernel (fire_finalize_kernel) then sums all partials in a fixed order and
+// advances the FIRE scalars on the device (energyMinimizerFIRE.cpp:77-145): no
+// host round trip.
 #include "qmin_device.h"
 #include "ldg_math.h"
 
@@ -87,13 +88,13 @@ __device__ __forceinline__ void nbr_derivative(int tn, int tu, int td, const R* 
 // Central-difference derivatives of the six face neighbours of the LDS centre
 // h (slot sc), transverse axes only, in the layout of the stencil tables'
 // derivative inputs x[35 + 15 d + 5 axis + c] (along-axis entries zero).
-template <typename R>
-__device__ __forceinline__ void central_nbr_derivatives(const R (*q)[5][kHP], int sm, int sc, int sp, int h, R* x) {
+template <int HX, int HP, typename R>
+__device__ __forceinline__ void central_nbr_derivatives(const R (*q)[5][HP], int sm, int sc, int sp, int h, R* x) {
 #pragma unroll
     for (int d = 0; d < 6; ++d) {
         const int axis = d / 2;
         const int sn = d == 4 ? sm : (d == 5 ? sp : sc);
-        const int hn = d == 0 ? h - 1 : d == 1 ? h + 1 : d == 2 ? h - kHX : d == 3 ? h + kHX : h;
+        const int hn = d == 0 ? h - 1 : d == 1 ? h + 1 : d == 2 ? h - HX : d == 3 ? h + HX : h;
 #pragma unroll
         for (int ax = 0; ax < 3; ++ax) {
             R* out = x + 35 + 15 * d + 5 * ax;
@@ -104,7 +105,7 @@ __device__ __forceinline__ void central_nbr_derivatives(const R (*q)[5][kHP], in
             }
             int su = sn, sd = sn, hu = hn, hd = hn;
             if (ax == 0) { hu = hn + 1; hd = hn - 1; }
-            if (ax == 1) { hu = hn + kHX; hd = hn - kHX; }
+            if (ax == 1) { hu = hn + HX; hd = hn - HX; }
             if (ax == 2) { su = sp; sd = sm; }
 #pragma unroll
             for (int c = 0; c < 5; ++c) out[c] = R(0.5) * (q[su][c][hu] - q[sd][c][hd]);
@@ -122,8 +123,8 @@ __device__ __forceinline__ void central_nbr_derivatives(const R (*q)[5][kHP], in
 template <typename R>
 using ConstPtr = const R __attribute__((address_space(4)))*;
 
-template <typename R, int TERMS>
-__device__ __forceinline__ void fast_raw_force(const ForceConst& fc, ConstPtr<R> W, const R (*q)[5][kHP],
+template <typename R, int TERMS, int HX, int HP>
+__device__ __forceinline__ void fast_raw_force(const ForceConst& fc, ConstPtr<R> W, const R (*q)[5][HP],
                                                int sm, int sc, int sp, int h, const R* qc, R* F) {
 #pragma unroll
     for (int c = 0; c < 5; ++c) F[c] = R(0);
@@ -132,19 +133,19 @@ __device__ __forceinline__ void fast_raw_force(const ForceConst& fc, ConstPtr<R>
     if constexpr (TERMS == kOneConstant) {
 #pragma unroll
         for (int c = 0; c < 5; ++c)
-            spatial[c] = R(fc.L1) * (R(6) * qc[c] - q[sc][c][h - 1] - q[sc][c][h + 1] - q[sc][c][h - kHX] -
-                                     q[sc][c][h + kHX] - q[sm][c][h] - q[sp][c][h]);
+            spatial[c] = R(fc.L1) * (R(6) * qc[c] - q[sc][c][h - 1] - q[sc][c][h + 1] - q[sc][c][h - HX] -
+                                     q[sc][c][h + HX] - q[sm][c][h] - q[sp][c][h]);
         l1_shape(spatial);
     } else if constexpr (TERMS == kMultiLinear) {
-        ldg_fast_lin_a(W, q, sm, sc, sp, h, spatial);
+        ldg_fast_lin_a<HX, HP>(W, q, sm, sc, sp, h, spatial);
     } else {
-        ldg_fast_lin_b(W, q, sm, sc, sp, h, spatial);
+        ldg_fast_lin_b<HX, HP>(W, q, sm, sc, sp, h, spatial);
     }
     if constexpr (TERMS == kMultiL6) {
         R x[125];
 #pragma unroll
         for (int c = 0; c < 5; ++c) x[c] = qc[c];
-        central_nbr_derivatives(q, sm, sc, sp, h, x);
+        central_nbr_derivatives<HX, HP>(q, sm, sc, sp, h, x);
         R o6[5];
         ldg_bulk_l6(x, o6);
 #pragma unroll
@@ -164,8 +165,18 @@ __device__ __forceinline__ void finish_bulk(const ForceConst& fc, R* F) {
     metric(F);
 }
 
-template <typename R, int MODE, int TERMS>
+// Stores of the next state: not read again in this pass, so they are streamed
+// past L2 (non-temporal), which keeps L2 for the halo rows that neighbouring
+// tiles read.
+template <typename R>
+__device__ __forceinline__ void st_out(R* p, R v) {
+    __builtin_nontemporal_store(v, p);
+}
+
+template <typename R, int MODE, int TERMS, int TILE>
 __global__ void __launch_bounds__(kThreads) march_kernel(MarchArgs<R> A) {
+    using T = Tile<kTileTX[TILE], kTileTY[TILE]>;
+    constexpr int kTX = T::TX, kTY = T::TY, kHX = T::HX, kHY = T::HY, kHP = T::HP, kHalo = T::Halo;
     __shared__ R sQ[3][5][kHP];
     __shared__ double sRed[kThreads / 64][3];
 
@@ -296,7 +307,8 @@ __global__ void __launch_bounds__(kThreads) march_kernel(MarchArgs<R> A) {
                     // inside the loop instead of being hoisted and spilled
                     uint64_t wa = reinterpret_cast<uint64_t>(A.W);
                     asm volatile("" : "+s"(wa));
-                    fast_raw_force<R, TERMS>(A.fc, reinterpret_cast<ConstPtr<R>>(wa), sQ, sm, sc, sp, hOwn, q, F);
+                    fast_raw_force<R, TERMS, kHX, kHP>(A.fc, reinterpret_cast<ConstPtr<R>>(wa), sQ, sm, sc, sp, hOwn,
+                                                       q, F);
                     finish_bulk(A.fc, F);
                 } else {
 #pragma unroll
@@ -308,9 +320,9 @@ __global__ void __launch_bounds__(kThreads) march_kernel(MarchArgs<R> A) {
                     for (int c = 0; c < 5; ++c) u[c] = vhCur[c] + hdt * F[c];
 #pragma unroll
                     for (int c = 0; c < 5; ++c) {
-                        out.q[c * n + i] = q[c];
-                        out.u[c * n + i] = u[c];
-                        out.f[c * n + i] = F[c];
+                        st_out(&out.q[c * n + i], q[c]);
+                        st_out(&out.u[c * n + i], u[c]);
+                        st_out(&out.f[c * n + i], F[c]);
                     }
                     pF += qt_dot(F, F);
                     pP += qt_dot(F, u);
@@ -323,8 +335,8 @@ __global__ void __launch_bounds__(kThreads) march_kernel(MarchArgs<R> A) {
                 // the boundary pass finishes U and F of this site
 #pragma unroll
                 for (int c = 0; c < 5; ++c) {
-                    out.q[c * n + i] = q[c];
-                    out.u[c * n + i] = vhCur[c];
+                    st_out(&out.q[c * n + i], q[c]);
+                    st_out(&out.u[c * n + i], vhCur[c]);
                 }
             }
         }
@@ -405,7 +417,6 @@ __device__ void fire_scalar_step(Ctrl* ctrl, const FireConst& fp, double fnorm, 
 template <typename R, int MODE, int TERMS>
 __global__ void __launch_bounds__(kBoundaryThreads) boundary_kernel(BoundaryArgs<R> A) {
     __shared__ double sRed[kBoundaryThreads / 64][3];
-    __shared__ int sLast;
     Ctrl* ctrl = A.ctrl;
     if ((MODE == kFire || A.gate) && ctrl->done) return;
     const int cur = ctrl->cur;
@@ -562,28 +573,26 @@ __global__ void __launch_bounds__(kBoundaryThreads) boundary_kernel(BoundaryArgs
         A.partials[3 * slot + 0] = pF;
         A.partials[3 * slot + 1] = pP;
         A.partials[3 * slot + 2] = pV;
-        __threadfence();
-        const unsigned prev = __hip_atomic_fetch_add(&ctrl->arrive, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        sLast = prev == gridDim.x - 1;
     }
-    __syncthreads();
-    if (!sLast) return;
-    __threadfence();
-    // fixed-order sum of every partial: march blocks then boundary blocks
-    const long long total = A.marchBlocks + gridDim.x;
+}
+
+// FIRE scalar step after both lattice passes of an iteration: one work-group
+// sums every partial in a fixed order (fused-pass blocks, then boundary-pass
+// blocks) and advances the FIRE scalars.  A kernel of its own: the launch
+// boundary makes the partials visible without device-wide fences, which cost
+// a write-back of each XCD's L2 right after the lattice pass has filled it.
+__global__ void __launch_bounds__(kReduceThreads) fire_finalize_kernel(Ctrl* ctrl, FireConst fp, const double* part,
+                                                                      long long total, double* trace) {
+    __shared__ double sRed[kReduceThreads / 64][3];
+    if (ctrl->done) return;
     double s0 = 0, s1 = 0, s2 = 0;
-    const double* part = A.partials;
-    for (long long j = threadIdx.x; j < total; j += kBoundaryThreads) {
+    for (long long j = threadIdx.x; j < total; j += kReduceThreads) {
         s0 += part[3 * j + 0];
         s1 += part[3 * j + 1];
         s2 += part[3 * j + 2];
     }
-    __syncthreads();
-    block_sum3<kBoundaryThreads>(s0, s1, s2, sRed);
-    if (threadIdx.x == 0) {
-        ctrl->arrive = 0;
-        fire_scalar_step(ctrl, A.fire, s0, s1, s2, A.trace);
-    }
+    block_sum3<kReduceThreads>(s0, s1, s2, sRed);
+    if (threadIdx.x == 0) fire_scalar_step(ctrl, fp, s0, s1, s2, trace);
 }
 
 // ---------------------------------------------------------------- other passes
@@ -718,8 +727,24 @@ __global__ void defect_kernel(const R* q, const int8_t* types, long long n, int 
 // ---------------------------------------------------------------- host launchers
 
 template <typename R, int MODE, int TERMS>
-static void launch_march_t(const MarchArgs<R>& a, hipStream_t s) {
-    hipLaunchKernelGGL((march_kernel<R, MODE, TERMS>), dim3(a.nblocks), dim3(kThreads), 0, s, a);
+static const void* march_fn_tile(int tile) {
+    return tile == kTile64x4 ? reinterpret_cast<const void*>(&march_kernel<R, MODE, TERMS, kTile64x4>)
+                             : reinterpret_cast<const void*>(&march_kernel<R, MODE, TERMS, kTile32x8>);
+}
+
+template <typename R, int MODE>
+static const void* march_fn_terms(int terms, int tile) {
+    switch (terms) {
+        case kOneConstant: return march_fn_tile<R, MODE, kOneConstant>(tile);
+        case kMultiLinear: return march_fn_tile<R, MODE, kMultiLinear>(tile);
+        case kMultiL4: return march_fn_tile<R, MODE, kMultiL4>(tile);
+        default: return march_fn_tile<R, MODE, kMultiL6>(tile);
+    }
+}
+
+template <typename R>
+static const void* march_fn(int mode, int terms, int tile) {
+    return mode == kFire ? march_fn_terms<R, kFire>(terms, tile) : march_fn_terms<R, kForce>(terms, tile);
 }
 
 template <typename R, int MODE, int TERMS>
@@ -728,18 +753,9 @@ static void launch_boundary_t(const BoundaryArgs<R>& a, int blocks, hipStream_t 
 }
 
 template <typename R>
-void launch_march(const MarchArgs<R>& a, int mode, int terms, hipStream_t s) {
-    if (mode == kFire) {
-        if (terms == kOneConstant) launch_march_t<R, kFire, kOneConstant>(a, s);
-        else if (terms == kMultiLinear) launch_march_t<R, kFire, kMultiLinear>(a, s);
-        else if (terms == kMultiL4) launch_march_t<R, kFire, kMultiL4>(a, s);
-        else launch_march_t<R, kFire, kMultiL6>(a, s);
-    } else {
-        if (terms == kOneConstant) launch_march_t<R, kForce, kOneConstant>(a, s);
-        else if (terms == kMultiLinear) launch_march_t<R, kForce, kMultiLinear>(a, s);
-        else if (terms == kMultiL4) launch_march_t<R, kForce, kMultiL4>(a, s);
-        else launch_march_t<R, kForce, kMultiL6>(a, s);
-    }
+hipError_t launch_march(const MarchArgs<R>& a, int mode, int terms, int tile, hipStream_t s) {
+    void* args[] = {const_cast<MarchArgs<R>*>(&a)};
+    return hipLaunchKernel(march_fn<R>(mode, terms, tile), dim3(a.nblocks), dim3(kThreads), args, 0, s);
 }
 
 template <typename R>
@@ -761,16 +777,17 @@ void launch_boundary(const BoundaryArgs<R>& a, int mode, int terms, hipStream_t 
 
 // Work-groups of the fused pass that fit on one CU at once.
 template <typename R>
-int march_blocks_per_cu(int mode, int terms) {
+int march_blocks_per_cu(int mode, int terms, int tile) {
     int blocks = 0;
-    const void* fn = nullptr;
-#define QMIN_PICK(M, T) if (mode == M && terms == T) fn = reinterpret_cast<const void*>(&march_kernel<R, M, T>);
-    QMIN_PICK(kFire, kOneConstant) QMIN_PICK(kFire, kMultiLinear) QMIN_PICK(kFire, kMultiL4) QMIN_PICK(kFire, kMultiL6)
-    QMIN_PICK(kForce, kOneConstant) QMIN_PICK(kForce, kMultiLinear) QMIN_PICK(kForce, kMultiL4)
-    QMIN_PICK(kForce, kMultiL6)
-#undef QMIN_PICK
-    if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fn, kThreads, 0) != hipSuccess) return 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, march_fn<R>(mode, terms, tile), kThreads, 0) !=
+        hipSuccess)
+        return 1;
     return blocks < 1 ? 1 : blocks;
+}
+
+void launch_fire_finalize(Ctrl* ctrl, const FireConst& fp, const double* partials, long long total, double* trace,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(fire_finalize_kernel, dim3(1), dim3(kReduceThreads), 0, s, ctrl, fp, partials, total, trace);
 }
 
 int boundary_blocks(long long nsites) {
@@ -804,13 +821,13 @@ void launch_defects(const R* q, const int8_t* types, long long n, int kind, doub
 }
 
 #define QMIN_INSTANTIATE(R)                                                                                      \
-    template void launch_march<R>(const MarchArgs<R>&, int, int, hipStream_t);                                   \
+    template hipError_t launch_march<R>(const MarchArgs<R>&, int, int, int, hipStream_t);                                   \
     template void launch_boundary<R>(const BoundaryArgs<R>&, int, int, hipStream_t);                             \
     template void launch_materialize<R>(StateSet<R>, StateSet<R>, long long, Ctrl*, hipStream_t);                \
     template int launch_energy<R>(const R*, const int8_t*, const DevObject*, int, int, int, const EnergyConst&,   \
                                   double*, hipStream_t);                                                        \
     template void launch_defects<R>(const R*, const int8_t*, long long, int, double*, hipStream_t);             \
-    template int march_blocks_per_cu<R>(int, int);
+    template int march_blocks_per_cu<R>(int, int, int);
 
 QMIN_INSTANTIATE(double)
 QMIN_INSTANTIATE(float)

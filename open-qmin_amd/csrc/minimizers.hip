@@ -4,11 +4,13 @@
 // force mode plus the boundary pass) and streaming kernels over the 5 n
 // components of Q, F and the minimizer's own vectors (same plane layout as Q).
 // Every streaming kernel reduces the dot products it needs into per-work-group
-// partials; the last work-group to arrive sums them in a fixed order and
+// partials; a one-work-group finalize kernel sums them in a fixed order and
 // finishes the minimizer's scalar logic in device memory (step coefficients,
-// forceMax, the stopping test).  A batch of iterations is therefore queued
-// without host round trips, every kernel after the stopping test is a no-op,
-// and the host reads the control block once per batch.
+// forceMax, the stopping test).  (A separate kernel, not a last-work-group
+// pattern: the device-wide fences that pattern needs write back each XCD's L2,
+// which the streaming kernel has just filled.)  A batch of iterations is
+// therefore queued without host round trips, every kernel after the stopping
+// test is a no-op, and the host reads the control block once per batch.
 //
 // Arithmetic follows the reference CPU steps: energyMinimizerAdam.cpp:25-54,
 // energyMinimizerNesterovAG.cpp:70-92 and energyMinimizerLoLBFGS.cpp:104-193.
@@ -55,38 +57,15 @@ __device__ __forceinline__ void block_sum(double (&v)[K], double (*red)[kMaxDots
     __syncthreads();
 }
 
-// Writes this work-group's partial sums; returns true in every thread of the
-// last work-group to arrive, whose thread 0 then holds the grid totals (summed
-// over work-groups in index order) in tot.
+// Writes this work-group's K partial sums (stride kMaxDots).
 template <int K>
-__device__ bool grid_sum(double (&v)[K], double* partials, unsigned* arrive, double (&tot)[K]) {
+__device__ __forceinline__ void write_partials(double (&v)[K], double* partials) {
     __shared__ double red[kVecThreads / 64][kMaxDots];
-    __shared__ int last;
     block_sum<K>(v, red);
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int k = 0; k < K; ++k) partials[K * blockIdx.x + k] = v[k];
-        __threadfence();
-        const unsigned prev = __hip_atomic_fetch_add(arrive, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        last = prev == gridDim.x - 1;
+        for (int k = 0; k < K; ++k) partials[kMaxDots * blockIdx.x + k] = v[k];
     }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
-    double s[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) s[k] = 0;
-    for (unsigned j = threadIdx.x; j < gridDim.x; j += kVecThreads) {
-#pragma unroll
-        for (int k = 0; k < K; ++k) s[k] += partials[K * j + k];
-    }
-    block_sum<K>(s, red);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < K; ++k) tot[k] = s[k];
-        *arrive = 0;
-    }
-    return true;
 }
 
 // Iteration bookkeeping shared by the three minimizers (thread 0 of the last
@@ -106,7 +85,6 @@ __device__ void finish_iteration(Ctrl* ctrl, double forceMax, double fc, double*
 struct Common {
     Ctrl* ctrl;
     double* partials;   // [blocks][kMaxDots]
-    unsigned* arrive;
     double* trace;      // [traceCap] forceMax per iteration
     double fc, ndof;
     long long len;      // 5 n
@@ -156,11 +134,8 @@ __global__ void __launch_bounds__(kVecThreads) adam_kernel(AdamArgs<R> A) {
             }
         }
     }
-    double part[1] = {ff}, tot[1];
-    if (!grid_sum<1>(part, A.c.partials, A.c.arrive, tot) || threadIdx.x != 0) return;
-    A.scal[0] = b1t * A.b1;
-    A.scal[1] = b2t * A.b2;
-    finish_iteration(ctrl, sqrt(tot[0]) / A.c.ndof, A.c.fc, A.c.trace);
+    double part[1] = {ff};
+    write_partials<1>(part, A.c.partials);
 }
 
 // ---------------------------------------------------------------- Nesterov
@@ -204,11 +179,8 @@ __global__ void __launch_bounds__(kVecThreads) nesterov_kernel(NesterovArgs<R> A
             }
         }
     }
-    double part[1] = {ff}, tot[1];
-    if (!grid_sum<1>(part, A.c.partials, A.c.arrive, tot) || threadIdx.x != 0) return;
-    A.scal[0] = lambda;
-    A.scal[1] = mud;
-    finish_iteration(ctrl, sqrt(tot[0]) / A.c.ndof, A.c.fc, A.c.trace);
+    double part[1] = {ff};
+    write_partials<1>(part, A.c.partials);
 }
 
 // ---------------------------------------------------------------- online L-BFGS
@@ -228,7 +200,6 @@ struct LbLayout {
     __host__ __device__ int size() const { return 5 * m + 1; }
 };
 
-enum LbOp { kOpAlpha, kOpBeta };
 
 template <typename R>
 struct LbStageArgs {
@@ -237,14 +208,13 @@ struct LbStageArgs {
     const R* x2;      // may be null
     const R* d;
     int k1, k2;       // scalar slots of the coefficients of x1, x2
-    int op, ii, t;    // finalisation: alpha_ii or alpha_ii - beta_ii of slot t
-    LbLayout lay;
     double* scal;
     Common c;
 };
 
-// p = (p + k1 x1) [+ k2 x2]; then d.p, finished into alpha_ii (step 2,
-// energyMinimizerLoLBFGS.cpp:117-131) or alpha_ii - beta_ii (step 4, :145-163).
+// p = (p + k1 x1) [+ k2 x2]; then partials of d.p, finished into alpha_ii
+// (step 2, energyMinimizerLoLBFGS.cpp:117-131) or alpha_ii - beta_ii (step 4,
+// :145-163) by the finalize kernel.
 template <typename R>
 __global__ void __launch_bounds__(kVecThreads) lb_stage_kernel(LbStageArgs<R> A) {
     Ctrl* ctrl = A.c.ctrl;
@@ -266,22 +236,8 @@ __global__ void __launch_bounds__(kVecThreads) lb_stage_kernel(LbStageArgs<R> A)
             }
         }
     }
-    double part[1] = {dp}, tot[1];
-    if (!grid_sum<1>(part, A.c.partials, A.c.arrive, tot) || threadIdx.x != 0) return;
-    const double sy = A.scal[A.lay.SY(A.t)];
-    if (A.op == kOpAlpha) {
-        double a = 0;
-        if (sy != 0) a = tot[0] / sy;
-        A.scal[A.lay.A(A.ii)] = a;
-        A.scal[A.lay.NA(A.ii)] = -a;
-    } else {
-        double cs = 0;
-        if (sy != 0) {
-            const double beta = tot[0] / sy;
-            cs = A.scal[A.lay.A(A.ii)] - beta;
-        }
-        A.scal[A.lay.CS(A.ii)] = cs;
-    }
+    double part[1] = {dp};
+    write_partials<1>(part, A.c.partials);
 }
 
 template <typename R>
@@ -327,16 +283,13 @@ struct LbFinishArgs {
     const R* f;
     const R* s;       // s of the newest slot
     R* y;             // y of the newest slot (null on the first call: no update)
-    int t;            // newest slot
-    int count;        // 1: an iteration ends here
-    LbLayout lay;
-    double* scal;
     Common c;
 };
 
-// p = F; y_t = y_t - F (:178-188); forceMax = sqrt(F.F)/N; caches s_t.y_t and
-// y_t.y_t, and starts the next two-loop pass: alpha_0 = s_t.p / s_t.y_t and the
-// step-3 factor (s.y)/(y.y) of the same slot (:117-142).
+// p = F; y_t = y_t - F (:178-188); partials of F.F, s_t.y_t, y_t.y_t and s_t.p,
+// from which the finalize kernel takes forceMax = sqrt(F.F)/N, caches s_t.y_t
+// and y_t.y_t, and starts the next two-loop pass: alpha_0 = s_t.p / s_t.y_t and
+// the step-3 factor (s.y)/(y.y) of the same slot (:117-142).
 template <typename R>
 __global__ void __launch_bounds__(kVecThreads) lb_finish_kernel(LbFinishArgs<R> A) {
     Ctrl* ctrl = A.c.ctrl;
@@ -363,20 +316,90 @@ __global__ void __launch_bounds__(kVecThreads) lb_finish_kernel(LbFinishArgs<R> 
             }
         }
     }
-    double part[4] = {pp, sy, yy, sp}, tot[4];
-    if (!grid_sum<4>(part, A.c.partials, A.c.arrive, tot) || threadIdx.x != 0) return;
-    const LbLayout& L = A.lay;
-    if (A.y) {
-        A.scal[L.SY(A.t)] = tot[1];
-        A.scal[L.YY(A.t)] = tot[2];
+    double part[4] = {pp, sy, yy, sp};
+    write_partials<4>(part, A.c.partials);
+}
+
+// ---------------------------------------------------------------- finalize
+enum FinOp { kFinAdam, kFinNesterov, kFinAlpha, kFinBeta, kFinLbFinish, kFinLbStart };
+
+struct FinArgs {
+    int op;
+    int blocks;       // work-groups of the streaming kernel whose partials are summed
+    int ii, t;        // two-loop index and memory slot
+    LbLayout lay;
+    double b1, b2;    // Adam
+    int scheduled;    // Nesterov
+    double* scal;
+    Common c;
+};
+
+// One work-group: fixed-order sums of the streaming kernel's partials, then the
+// minimizer's scalar step in thread 0.
+__global__ void __launch_bounds__(kVecThreads) vec_finalize_kernel(FinArgs A) {
+    __shared__ double red[kVecThreads / 64][kMaxDots];
+    Ctrl* ctrl = A.c.ctrl;
+    if (ctrl->done) return;
+    double tot[kMaxDots] = {0, 0, 0, 0};
+    for (int j = threadIdx.x; j < A.blocks; j += kVecThreads) {
+#pragma unroll
+        for (int k = 0; k < kMaxDots; ++k) tot[k] += A.c.partials[kMaxDots * j + k];
     }
-    const double syt = A.scal[L.SY(A.t)], yyt = A.scal[L.YY(A.t)];
-    double a = 0;
-    if (syt != 0) a = tot[3] / syt;
-    A.scal[L.A(0)] = a;
-    A.scal[L.NA(0)] = -a;
-    A.scal[L.K3()] = yyt != 0 ? syt / yyt : 0.0;
-    if (A.count) finish_iteration(ctrl, sqrt(tot[0]) / A.c.ndof, A.c.fc, A.c.trace);
+    block_sum<kMaxDots>(tot, red);
+    if (threadIdx.x != 0) return;
+    double* scal = A.scal;
+    const LbLayout& L = A.lay;
+    switch (A.op) {
+        case kFinAdam:
+            // beta^t advances once per executed step (energyMinimizerAdam.cpp:52-53)
+            scal[0] = scal[0] * A.b1;
+            scal[1] = scal[1] * A.b2;
+            finish_iteration(ctrl, sqrt(tot[0]) / A.c.ndof, A.c.fc, A.c.trace);
+            break;
+        case kFinNesterov: {
+            // the same lambda and momentum the step kernel computed from scal[0]
+            const double oldLambda = scal[0];
+            const double lambda = 0.5 * (1.0 + sqrt(1.0 + 4.0 * oldLambda * oldLambda));
+            const double gamma = (1 - oldLambda) / lambda;
+            if (A.scheduled) scal[1] = 1. + gamma;
+            scal[0] = lambda;
+            finish_iteration(ctrl, sqrt(tot[0]) / A.c.ndof, A.c.fc, A.c.trace);
+            break;
+        }
+        case kFinAlpha: {
+            const double sy = scal[L.SY(A.t)];
+            double a = 0;
+            if (sy != 0) a = tot[0] / sy;
+            scal[L.A(A.ii)] = a;
+            scal[L.NA(A.ii)] = -a;
+            break;
+        }
+        case kFinBeta: {
+            const double sy = scal[L.SY(A.t)];
+            double cs = 0;
+            if (sy != 0) {
+                const double beta = tot[0] / sy;
+                cs = scal[L.A(A.ii)] - beta;
+            }
+            scal[L.CS(A.ii)] = cs;
+            break;
+        }
+        case kFinLbFinish:
+        case kFinLbStart: {
+            if (A.op == kFinLbFinish) {
+                scal[L.SY(A.t)] = tot[1];
+                scal[L.YY(A.t)] = tot[2];
+            }
+            const double syt = scal[L.SY(A.t)], yyt = scal[L.YY(A.t)];
+            double a = 0;
+            if (syt != 0) a = tot[3] / syt;
+            scal[L.A(0)] = a;
+            scal[L.NA(0)] = -a;
+            scal[L.K3()] = yyt != 0 ? syt / yyt : 0.0;
+            if (A.op == kFinLbFinish) finish_iteration(ctrl, sqrt(tot[0]) / A.c.ndof, A.c.fc, A.c.trace);
+            break;
+        }
+    }
 }
 
 int vec_blocks(long long len) {
@@ -401,7 +424,6 @@ struct VecMinimizer {
     double* dScal = nullptr;
     int scalCount = 0;
     double* dPartials = nullptr;
-    unsigned* dArrive = nullptr;
     double* dTrace = nullptr;
     int dTraceCap = 0;
     int traceCap = 0;
@@ -420,10 +442,8 @@ struct VecMinimizer {
         vecs.clear();
         if (dScal) (void)hipFree(dScal);
         if (dPartials) (void)hipFree(dPartials);
-        if (dArrive) (void)hipFree(dArrive);
         dScal = nullptr;
         dPartials = nullptr;
-        dArrive = nullptr;
         bound = nullptr;
         boundN = 0;
         boundPrecision = -1;
@@ -448,8 +468,6 @@ struct VecMinimizer {
         HIP_TRY(hipMemsetAsync(dScal, 0, std::max(nscal, 1) * sizeof(double), L->stream));
         scalCount = nscal;
         HIP_TRY(hipMalloc(&dPartials, (size_t)kVecMaxBlocks * kMaxDots * sizeof(double)));
-        HIP_TRY(hipMalloc(&dArrive, sizeof(unsigned)));
-        HIP_TRY(hipMemsetAsync(dArrive, 0, sizeof(unsigned), L->stream));
         bound = L;
         boundN = L->n;
         boundPrecision = L->precision;
@@ -467,7 +485,6 @@ struct VecMinimizer {
         Common c{};
         c.ctrl = L->dCtrl;
         c.partials = dPartials;
-        c.arrive = dArrive;
         c.trace = traceCap > 0 ? dTrace : nullptr;
         c.fc = fc;
         c.ndof = (double)L->n;
@@ -476,6 +493,15 @@ struct VecMinimizer {
     }
     int set_scalars(const qm_lattice* L, const double* v, int count, int offset = 0) {
         HIP_TRY(hipMemcpyAsync(dScal + offset, v, count * sizeof(double), hipMemcpyHostToDevice, L->stream));
+        return QM_OK;
+    }
+    // the finalize kernel for a streaming kernel of `blocks` work-groups
+    int finalize(const qm_lattice* L, FinArgs a, int blocks) const {
+        a.blocks = blocks;
+        a.scal = dScal;
+        a.c = common(L);
+        vec_finalize_kernel<<<1, kVecThreads, 0, L->stream>>>(a);
+        HIP_TRY(hipGetLastError());
         return QM_OK;
     }
     int get_scalars(const qm_lattice* L, double* v, int count, int offset = 0) {
@@ -561,9 +587,14 @@ static int adam_enqueue(qm_adam* m, qm_lattice* L, const qm_ldg* f) {
     a.eps = m->eps;
     a.dt = m->dt;
     a.c = m->common(L);
-    adam_kernel<R><<<vec_blocks(a.c.len), kVecThreads, 0, L->stream>>>(a);
+    const int blocks = vec_blocks(a.c.len);
+    adam_kernel<R><<<blocks, kVecThreads, 0, L->stream>>>(a);
     HIP_TRY(hipGetLastError());
-    return QM_OK;
+    FinArgs fa{};
+    fa.op = kFinAdam;
+    fa.b1 = m->b1;
+    fa.b2 = m->b2;
+    return m->finalize(L, fa, blocks);
 }
 
 extern "C" {
@@ -649,9 +680,13 @@ static int nesterov_enqueue(qm_nesterov* m, qm_lattice* L, const qm_ldg* f) {
     a.dt = m->dt;
     a.scheduled = m->scheduled;
     a.c = m->common(L);
-    nesterov_kernel<R><<<vec_blocks(a.c.len), kVecThreads, 0, L->stream>>>(a);
+    const int blocks = vec_blocks(a.c.len);
+    nesterov_kernel<R><<<blocks, kVecThreads, 0, L->stream>>>(a);
     HIP_TRY(hipGetLastError());
-    return QM_OK;
+    FinArgs fa{};
+    fa.op = kFinNesterov;
+    fa.scheduled = m->scheduled;
+    return m->finalize(L, fa, blocks);
 }
 
 extern "C" {
@@ -749,13 +784,14 @@ static int lb_enqueue(qm_lolbfgs* M, qm_lattice* L, const qm_ldg* f, int iterati
         a.f = st.f;
         a.s = rp<R>(M->s(slot(0)));
         a.y = nullptr;
-        a.t = slot(0);
-        a.count = 0;
-        a.lay = lay;
-        a.scal = M->dScal;
         a.c = com;
         lb_finish_kernel<R><<<blocks, kVecThreads, 0, L->stream>>>(a);
         HIP_TRY(hipGetLastError());
+        FinArgs fa{};
+        fa.op = kFinLbStart;
+        fa.t = slot(0);
+        fa.lay = lay;
+        if (int rc = M->finalize(L, fa, blocks)) return rc;
     }
     auto stage = [&](const R* x1, int k1, const R* x2, int k2, const R* d, int op, int ii, int t) {
         LbStageArgs<R> a{};
@@ -765,27 +801,32 @@ static int lb_enqueue(qm_lolbfgs* M, qm_lattice* L, const qm_ldg* f, int iterati
         a.d = d;
         a.k1 = k1;
         a.k2 = k2;
-        a.op = op;
-        a.ii = ii;
-        a.t = t;
-        a.lay = lay;
         a.scal = M->dScal;
         a.c = com;
         lb_stage_kernel<R><<<blocks, kVecThreads, 0, L->stream>>>(a);
-        return hipGetLastError();
+        HIP_TRY(hipGetLastError());
+        FinArgs fa{};
+        fa.op = op;
+        fa.ii = ii;
+        fa.t = t;
+        fa.lay = lay;
+        return M->finalize(L, fa, blocks);
     };
     // step 2: p -= alpha_ii y; alpha_{ii+1} = s.p / s.y (newest first)
     for (int ii = 0; ii + 1 < m; ++ii)
-        HIP_TRY(stage(rp<R>(M->y(slot(ii))), lay.NA(ii), nullptr, 0, rp<R>(M->s(slot(ii + 1))), kOpAlpha, ii + 1,
-                      slot(ii + 1)));
+        if (int rc = stage(rp<R>(M->y(slot(ii))), lay.NA(ii), nullptr, 0, rp<R>(M->s(slot(ii + 1))), kFinAlpha,
+                           ii + 1, slot(ii + 1)))
+            return rc;
     // last step-2 update fused with step 3 (p += (s.y / y.y) y of the newest
     // slot), then beta of the oldest slot
-    HIP_TRY(stage(rp<R>(M->y(slot(m - 1))), lay.NA(m - 1), rp<R>(M->y(slot(0))), lay.K3(),
-                  rp<R>(M->y(slot(m - 1))), kOpBeta, m - 1, slot(m - 1)));
+    if (int rc = stage(rp<R>(M->y(slot(m - 1))), lay.NA(m - 1), rp<R>(M->y(slot(0))), lay.K3(),
+                       rp<R>(M->y(slot(m - 1))), kFinBeta, m - 1, slot(m - 1)))
+        return rc;
     // step 4: p += (alpha_ii - beta_ii) s; beta_{ii-1} = y.p / s.y (oldest first)
     for (int ii = m - 1; ii >= 1; --ii)
-        HIP_TRY(stage(rp<R>(M->s(slot(ii))), lay.CS(ii), nullptr, 0, rp<R>(M->y(slot(ii - 1))), kOpBeta, ii - 1,
-                      slot(ii - 1)));
+        if (int rc = stage(rp<R>(M->s(slot(ii))), lay.CS(ii), nullptr, 0, rp<R>(M->y(slot(ii - 1))), kFinBeta,
+                           ii - 1, slot(ii - 1)))
+            return rc;
     // update step: eta as minimize() sets it (energyMinimizerLoLBFGS.cpp:203-209)
     const double eta = M->scheduled ? M->tau / (M->tau + iteration) * M->dt : M->dt;
     {
@@ -810,13 +851,14 @@ static int lb_enqueue(qm_lolbfgs* M, qm_lattice* L, const qm_ldg* f, int iterati
         a.f = st.f;
         a.s = rp<R>(M->s(c));
         a.y = rp<R>(M->y(c));
-        a.t = c;
-        a.count = 1;
-        a.lay = lay;
-        a.scal = M->dScal;
         a.c = com;
         lb_finish_kernel<R><<<blocks, kVecThreads, 0, L->stream>>>(a);
         HIP_TRY(hipGetLastError());
+        FinArgs fa{};
+        fa.op = kFinLbFinish;
+        fa.t = c;
+        fa.lay = lay;
+        if (int rc = M->finalize(L, fa, blocks)) return rc;
     }
     M->cur = (c + 1) % m;
     return QM_OK;

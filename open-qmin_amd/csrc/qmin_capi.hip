@@ -19,12 +19,14 @@
 
 namespace qmin {
 template <typename R>
-void launch_march(const MarchArgs<R>& a, int mode, int terms, hipStream_t s);
+hipError_t launch_march(const MarchArgs<R>& a, int mode, int terms, int tile, hipStream_t s);
 template <typename R>
 void launch_boundary(const BoundaryArgs<R>& a, int mode, int terms, hipStream_t s);
 int boundary_blocks(long long nsites);
+void launch_fire_finalize(Ctrl* ctrl, const FireConst& fp, const double* partials, long long total, double* trace,
+                          hipStream_t s);
 template <typename R>
-int march_blocks_per_cu(int mode, int terms);
+int march_blocks_per_cu(int mode, int terms, int tile);
 template <typename R>
 void launch_materialize(StateSet<R> s0, StateSet<R> s1, long long n, Ctrl* ctrl, hipStream_t s);
 template <typename R>
@@ -276,6 +278,12 @@ int qm_lattice_set_precision(qm_lattice* L, int precision) {
     }
     L->precision = precision;
     L->realSize = realSize;
+    return QM_OK;
+}
+
+int qm_lattice_set_tile(qm_lattice* L, int tile) {
+    if (!L || tile < -1 || tile >= kNumTiles) return fail(QM_EINVAL, "tile is -1 (automatic), 0 (32x8) or 1 (64x4)");
+    L->tileOverride = tile;
     return QM_OK;
 }
 
@@ -665,6 +673,17 @@ static int upload_weights(qm_lattice* L, const qm_ldg* f) {
     return QM_OK;
 }
 
+// Tile shape of the fused pass: the one set on the lattice, else the measured
+// better default per precision (32 x 8 for FP64, 64 x 4 for FP32).
+static int active_tile(const qm_lattice* L) {
+    if (L->tileOverride >= 0) return L->tileOverride;
+    return L->precision == QM_FP64 ? kTile32x8 : kTile64x4;
+}
+
+static long long tile_columns(const qm_lattice* L, int tile) {
+    return (long long)((L->lx + kTileTX[tile] - 1) / kTileTX[tile]) * ((L->ly + kTileTY[tile] - 1) / kTileTY[tile]);
+}
+
 template <typename R>
 static void fill_args(qm_lattice* L, const qm_ldg* f, MarchArgs<R>& ma, BoundaryArgs<R>& ba, const FireConst& fire) {
     ma.set[0] = state_set<R>(L, 0);
@@ -676,8 +695,9 @@ static void fill_args(qm_lattice* L, const qm_ldg* f, MarchArgs<R>& ma, Boundary
     ma.lz = L->lz;
     ma.zchunk = std::min(L->zchunk, L->lz);
     ma.n = L->n;
-    ma.nbx = (L->lx + kTX - 1) / kTX;
-    ma.nby = (L->ly + kTY - 1) / kTY;
+    const int tile = active_tile(L);
+    ma.nbx = (L->lx + kTileTX[tile] - 1) / kTileTX[tile];
+    ma.nby = (L->ly + kTileTY[tile] - 1) / kTileTY[tile];
     const int nbz = (L->lz + ma.zchunk - 1) / ma.zchunk;
     ma.nblocks = ma.nbx * ma.nby * nbz;
     ma.fc = force_const(f);
@@ -717,9 +737,10 @@ static void choose_zchunk(qm_lattice* L, int mode, int terms) {
             L->numCUs = prop.multiProcessorCount;
         if (L->numCUs <= 0) L->numCUs = 256;
     }
-    const int perCU = L->precision == QM_FP64 ? march_blocks_per_cu<double>(mode, terms)
-                                              : march_blocks_per_cu<float>(mode, terms);
-    const long long columns = (long long)((L->lx + kTX - 1) / kTX) * ((L->ly + kTY - 1) / kTY);
+    const int tile = active_tile(L);
+    const int perCU = L->precision == QM_FP64 ? march_blocks_per_cu<double>(mode, terms, tile)
+                                              : march_blocks_per_cu<float>(mode, terms, tile);
+    const long long columns = tile_columns(L, tile);
     long long chunks = (long long)perCU * L->numCUs / columns;
     if (chunks < 1) chunks = 1;
     int zc = (int)((L->lz + chunks - 1) / chunks);
@@ -728,7 +749,7 @@ static void choose_zchunk(qm_lattice* L, int mode, int terms) {
 
 static int march_blocks(const qm_lattice* L) {
     const int zc = std::min(L->zchunk, L->lz);
-    return ((L->lx + kTX - 1) / kTX) * ((L->ly + kTY - 1) / kTY) * ((L->lz + zc - 1) / zc);
+    return int(tile_columns(L, active_tile(L)) * ((L->lz + zc - 1) / zc));
 }
 
 // One pass (force only or one FIRE iteration); kernels only, no host sync.
@@ -746,11 +767,15 @@ static int run_pass(qm_lattice* L, const qm_ldg* f, int mode, const FireConst& f
         for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
         HIP_TRY(hipEventRecord(ev[0], L->stream));
     }
-    launch_march<R>(ma, mode, terms, L->stream);
-    HIP_TRY(hipGetLastError());
+    HIP_TRY(launch_march<R>(ma, mode, terms, active_tile(L), L->stream));
     if (g_timing.on && mode == kFire) HIP_TRY(hipEventRecord(ev[1], L->stream));
     launch_boundary<R>(ba, mode, terms, L->stream);
     HIP_TRY(hipGetLastError());
+    if (mode == kFire) {
+        launch_fire_finalize(L->dCtrl, fire, L->dPartials, (long long)ma.nblocks + boundary_blocks(L->nBoundary),
+                             L->dTrace, L->stream);
+        HIP_TRY(hipGetLastError());
+    }
     if (g_timing.on && mode == kFire) {
         HIP_TRY(hipEventRecord(ev[2], L->stream));
         HIP_TRY(hipEventSynchronize(ev[2]));
@@ -771,7 +796,8 @@ int qmin::prepare(qm_lattice* L, const qm_ldg* f) {
     if (int rc = refresh_types(L)) return rc;
     if (int rc = upload_weights(L, f)) return rc;
     // room for the smallest z chunk (4 planes) the fused pass may pick
-    const long long worst = (long long)((L->lx + kTX - 1) / kTX) * ((L->ly + kTY - 1) / kTY) * ((L->lz + 3) / 4);
+    long long worst = 0;
+    for (int t = 0; t < kNumTiles; ++t) worst = std::max(worst, tile_columns(L, t) * ((L->lz + 3) / 4));
     const size_t need = 3 * (size_t)(worst + boundary_blocks(L->nBoundary));
     return ensure_partials(L, need);
 }

@@ -5,15 +5,25 @@
 
 namespace qmin {
 
-// Lattice tile of the fused pass: 32 x 8 sites per work-group (4 wave64), with a
-// one-site halo in x and y; the work-group marches along z through a chunk.
-constexpr int kTX = 32;
-constexpr int kTY = 8;
-constexpr int kThreads = kTX * kTY;
-constexpr int kHX = kTX + 2;
-constexpr int kHY = kTY + 2;
-constexpr int kHP = kHX * kHY;          // sites of one halo-padded plane
-constexpr int kHalo = kHP - kThreads;   // perimeter sites of one plane
+// Work-group of the fused pass: 256 threads (4 wave64), one site each, in a
+// TX x TY tile with a one-site halo in x and y; the work-group marches along z
+// through a chunk.  Two tile shapes are compiled: 32 x 8 and 64 x 4 (wider rows
+// use whole cache lines for the side halo; more halo sites per tile).
+constexpr int kThreads = 256;
+enum TileId { kTile32x8 = 0, kTile64x4 = 1, kNumTiles = 2 };
+constexpr int kTileTX[kNumTiles] = {32, 64};
+constexpr int kTileTY[kNumTiles] = {8, 4};
+
+template <int TX_, int TY_>
+struct Tile {
+    static constexpr int TX = TX_, TY = TY_;
+    static constexpr int HX = TX + 2, HY = TY + 2;
+    static constexpr int HP = HX * HY;          // sites of one halo-padded plane
+    static constexpr int Halo = HP - TX * TY;   // perimeter sites of one plane
+    static_assert(TX * TY == kThreads, "one site per thread");
+    static_assert(Halo <= kThreads, "one halo site per thread at most");
+};
+
 constexpr int kMaxObjects = 127;        // types are int8 on the device
 constexpr int kBoundaryThreads = 128;
 constexpr int kReduceThreads = 256;

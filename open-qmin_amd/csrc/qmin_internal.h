@@ -40,6 +40,7 @@ struct qm_lattice {
     bool typesDirty = true;
     int zchunk = 16;          // z extent one work-group marches through
     int zchunkOverride = 0;   // fixed z chunk (0 = choose from occupancy)
+    int tileOverride = -1;    // tile shape of the fused pass (-1 = by precision)
     int numCUs = 0;
 };
 

@@ -42,6 +42,7 @@ _API = {
     "qm_lattice_destroy": [_VP],
     "qm_lattice_dims": [_VP] + [ctypes.POINTER(ctypes.c_int)] * 4,
     "qm_lattice_set_zchunk": [_VP, _i],
+    "qm_lattice_set_tile": [_VP, _i],
     "qm_lattice_set_precision": [_VP, _i],
     "qm_lattice_set_field": [_VP, _i, _D],
     "qm_lattice_get_field": [_VP, _i, _D],
@@ -163,6 +164,10 @@ class Lattice:
 
     def set_zchunk(self, zchunk):
         _check(self.lib.qm_lattice_set_zchunk(self.h, zchunk))
+
+    def set_tile(self, tile):
+        """Tile shape of the fused pass: -1 automatic, 0 = 32x8, 1 = 64x4."""
+        _check(self.lib.qm_lattice_set_tile(self.h, tile))
 
     def set_precision(self, precision):
         """Convert the device state to QM_FP64 or QM_FP32 (e.g. an FP64 polish)."""

@@ -201,8 +201,8 @@ def gen_fast(tables):
         for k in keys:
             host.append("    {%s}," % ", ".join(repr(float(t.get(k, 0))) for t in terms))
         host.append("};")
-        dev += ["template <typename R, typename WP>",
-                "__device__ __forceinline__ void ldg_fast_lin_%s(WP W, const R (*q)[5][kHP], int sm,"
+        dev += ["template <int HX, int HP, typename R, typename WP>",
+                "__device__ __forceinline__ void ldg_fast_lin_%s(WP W, const R (*q)[5][HP], int sm,"
                 " int sc, int sp, int h, R* acc) {" % name,
                 "    R v;"]
         last = None
@@ -219,7 +219,7 @@ def gen_fast(tables):
                 if p[0]:
                     off += " %s 1" % ("+" if p[0] > 0 else "-")
                 if p[1]:
-                    off += " %s kHX" % ("+" if p[1] > 0 else "-")
+                    off += " %s HX" % ("+" if p[1] > 0 else "-")
                 dev.append("    v = q[%s][%d][%s];" % (slot, c, off))
                 last = (p, c)
             dev.append("    acc[%d] += W[%d] * v;" % (r, idx))

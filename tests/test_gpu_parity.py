@@ -175,3 +175,28 @@ def test_precision_conversion_round_trip():
     m.set_max_iterations(10)
     st = m.minimize(f, g)
     assert st.iterations == 10 and np.isfinite(st.forceMax)
+
+
+@pytest.mark.parametrize("precision", [qmin.QM_FP64, qmin.QM_FP32])
+def test_tile_shapes_agree(precision):
+    """Both compiled tile shapes of the fused pass give the same per-site forces
+    (bit for bit: the per-site arithmetic does not depend on the tiling) and FIRE
+    traces equal to reduction rounding; odd sizes exercise partial tiles."""
+    shape = (70, 11, 9)
+    o, g, f = build_pair(shape, MULTI, 1, True, precision=precision)
+    got = []
+    for tile in (0, 1):
+        g.set_tile(tile)
+        got.append(f.compute_forces(g))
+    assert np.array_equal(got[0], got[1])
+    if precision == qmin.QM_FP64:
+        assert_forces_close(got[0], o.forces())
+    traces = []
+    for tile in (0, 1):
+        o2, g2, f2 = build_pair(shape, MULTI, 1, True, precision=precision)
+        g2.set_tile(tile)
+        m = qmin.FIRE(max_iter=30)
+        m.set_trace(32)
+        m.minimize(f2, g2)
+        traces.append(m.trace())
+    np.testing.assert_allclose(traces[0][:, 0], traces[1][:, 0], rtol=1e-12 if precision == qmin.QM_FP64 else 1e-6)
