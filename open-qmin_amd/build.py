@@ -57,6 +57,10 @@ def build(verbose=False):
          os.path.join(HERE, "host", "openqmin_hip.cpp"), "-o", DRIVER + ".tmp", "-L" + os.path.dirname(LIB),
          "-lqmin", "-Wl,-rpath,$ORIGIN"])
     os.replace(DRIVER + ".tmp", DRIVER)
+    # streaming-ceiling measurement for the fused pass's access pattern
+    ceiling = os.path.join(os.path.dirname(LIB), "qmin-stream-ceiling")
+    run([HIPCC] + FLAGS + [os.path.join(CSRC, "stream_ceiling.hip"), "-o", ceiling + ".tmp"])
+    os.replace(ceiling + ".tmp", ceiling)
     return LIB
 
 
