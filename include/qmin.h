@@ -149,6 +149,9 @@ int qm_fire_set_parameters(qm_fire* m, double dt, double alphaStart, double dtMa
                            double alphaDec, int nMin, double forceCutoff, double alphaMin);
 /* updater::setMaximumIterations (baseUpdater.h:77) */
 int qm_fire_set_max_iterations(qm_fire* m, int maxIterations);
+/* energyMinimizerFIRE::setForceCutoff (energyMinimizerFIRE.h:29): the other
+ * FIRE state (dt, alpha, N+) is kept, e.g. for an FP64 polish to a tighter cutoff */
+int qm_fire_set_force_cutoff(qm_fire* m, double forceCutoff);
 /* energyMinimizerFIRE::minimize (energyMinimizerFIRE.cpp:151-170) on the device:
  * forces, then fused velocity-Verlet + force + FIRE iterations until the tolerance
  * or the iteration cap.  The velocity field persists in the lattice. */

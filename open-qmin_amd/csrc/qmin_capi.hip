@@ -911,6 +911,12 @@ int qm_fire_set_parameters(qm_fire* m, double dt, double alphaStart, double dtMa
     return QM_OK;
 }
 
+int qm_fire_set_force_cutoff(qm_fire* m, double fc) {
+    if (!m) return fail(QM_EINVAL, "null minimizer");
+    m->fc = fc;
+    return QM_OK;
+}
+
 int qm_fire_set_max_iterations(qm_fire* m, int it) {
     if (!m) return fail(QM_EINVAL, "null minimizer");
     m->maxIter = it;

@@ -293,6 +293,8 @@ class energyMinimizerFIRE : public updater {
                                                 nMin, forceCutoff, alphaMin),
                          "setFIREParameters");
     }
+    // energyMinimizerFIRE::setForceCutoff (energyMinimizerFIRE.h:29)
+    void setForceCutoff(scalar fc) { qmin_host::check(qm_fire_set_force_cutoff(m_, fc), "setForceCutoff"); }
     void minimize() {
         qmin_host::check(qm_fire_set_max_iterations(m_, maxIterations), "setMaximumIterations");
         qmin_host::check(qm_fire_minimize(m_, force(), lattice()), "energyMinimizerFIRE::minimize");

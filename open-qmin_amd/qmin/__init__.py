@@ -73,6 +73,7 @@ _API = {
     "qm_fire_destroy": [_VP],
     "qm_fire_set_parameters": [_VP, _d, _d, _d, _d, _d, _d, _i, _d, _d],
     "qm_fire_set_max_iterations": [_VP, _i],
+    "qm_fire_set_force_cutoff": [_VP, _d],
     "qm_fire_minimize": [_VP, _VP, _VP],
     "qm_fire_run_iterations": [_VP, _VP, _VP, _i],
     "qm_fire_get_state": [_VP, ctypes.POINTER(FireState)],
@@ -328,6 +329,10 @@ class FIRE:
 
     def set_max_iterations(self, n):
         _check(self.lib.qm_fire_set_max_iterations(self.h, n))
+
+    def set_force_cutoff(self, fc):
+        """energyMinimizerFIRE::setForceCutoff; dt, alpha and N+ are kept."""
+        _check(self.lib.qm_fire_set_force_cutoff(self.h, fc))
 
     def set_trace(self, capacity):
         _check(self.lib.qm_fire_set_trace(self.h, capacity))
