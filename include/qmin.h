@@ -63,6 +63,12 @@ int qm_lattice_set_zchunk(qm_lattice* lat, int zchunk);
 /* tile shape of the fused lattice pass: -1 (default) picks by precision, 0 is
  * 32x8 sites per work-group, 1 is 64x4.  A tuning knob of this implementation. */
 int qm_lattice_set_tile(qm_lattice* lat, int tile);
+/* tile autotuner, the role of kernelTuner (src/utilities/kernelTuner.cpp:5-125):
+ * times the fused FIRE pass with HIP events for every tile shape and z chunks
+ * {automatic, 8, 16, 32, 64} on the lattice's current state (which it leaves
+ * unchanged), keeps the fastest on the lattice and returns it (any output may be
+ * NULL).  Explicit, so that a run's reduction order does not depend on timing. */
+int qm_lattice_autotune(qm_lattice* lat, qm_ldg* f, int* tile, int* zchunk, double* ms);
 /* converts the device state (Q, v, F) to the other storage precision, e.g. for an
  * FP64 polish after an FP32 minimization; minimizers bound to the lattice other
  * than FIRE restart their history.  No reference counterpart (the reference is

@@ -791,6 +791,76 @@ static int run_pass(qm_lattice* L, const qm_ldg* f, int mode, const FireConst& f
     return QM_OK;
 }
 
+// Average duration of the fused FIRE pass alone, on the current state: it
+// reads the current state set and writes only the other one and the partial
+// sums, which the next real pass overwrites, so timing it changes nothing.
+template <typename R>
+static int time_march(qm_lattice* L, const qm_ldg* f, int reps, double* ms) {
+    choose_zchunk(L, kFire, terms_of(f));
+    MarchArgs<R> ma{};
+    BoundaryArgs<R> ba{};
+    FireConst fire{};
+    fill_args<R>(L, f, ma, ba, fire);
+    const int terms = terms_of(f), tile = active_tile(L);
+    for (int w = 0; w < 2; ++w) HIP_TRY(launch_march<R>(ma, kFire, terms, tile, L->stream));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, L->stream));
+    for (int r = 0; r < reps; ++r) HIP_TRY(launch_march<R>(ma, kFire, terms, tile, L->stream));
+    HIP_TRY(hipEventRecord(e1, L->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float t = 0;
+    HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = t / reps;
+    return QM_OK;
+}
+
+// Tile autotuner (the role of the reference's kernelTuner, kernelTuner.cpp:5-125,
+// measured with HIP events on the library's stream instead of host clocks):
+// every compiled tile shape x z chunks {automatic, 8, 16, 32, 64}.
+int qm_lattice_autotune(qm_lattice* L, qm_ldg* f, int* tileOut, int* zchunkOut, double* msOut) {
+    if (int rc = prepare(L, f)) return rc;
+    if (int rc = sync_ctrl_to_host(L)) return rc;
+    const int savedDone = L->hCtrl.done;
+    if (savedDone) {
+        // the pass returns at once on a finished control block
+        L->hCtrl.done = 0;
+        if (int rc = push_ctrl(L)) return rc;
+    }
+    const int reps = L->n >= (1 << 22) ? 5 : 20;
+    double best = 1e30;
+    int bestTile = 0, bestZ = 0;
+    for (int tile = 0; tile < kNumTiles; ++tile) {
+        for (int zc : {0, 8, 16, 32, 64}) {
+            if (zc > L->lz) continue;
+            L->tileOverride = tile;
+            L->zchunkOverride = zc;
+            double ms = 0;
+            int rc = L->precision == QM_FP64 ? time_march<double>(L, f, reps, &ms)
+                                             : time_march<float>(L, f, reps, &ms);
+            if (rc) return rc;
+            if (ms < best) {
+                best = ms;
+                bestTile = tile;
+                bestZ = zc;
+            }
+        }
+    }
+    L->tileOverride = bestTile;
+    L->zchunkOverride = bestZ;
+    if (savedDone) {
+        L->hCtrl.done = savedDone;
+        if (int rc = push_ctrl(L)) return rc;
+    }
+    if (tileOut) *tileOut = bestTile;
+    if (zchunkOut) *zchunkOut = bestZ;
+    if (msOut) *msOut = best;
+    return QM_OK;
+}
+
 int qmin::prepare(qm_lattice* L, const qm_ldg* f) {
     if (!L || !f) return fail(QM_EINVAL, "null handle");
     if (int rc = refresh_types(L)) return rc;

@@ -43,6 +43,7 @@ _API = {
     "qm_lattice_dims": [_VP] + [ctypes.POINTER(ctypes.c_int)] * 4,
     "qm_lattice_set_zchunk": [_VP, _i],
     "qm_lattice_set_tile": [_VP, _i],
+    "qm_lattice_autotune": [_VP, _VP, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), _D],
     "qm_lattice_set_precision": [_VP, _i],
     "qm_lattice_set_field": [_VP, _i, _D],
     "qm_lattice_get_field": [_VP, _i, _D],
@@ -169,6 +170,13 @@ class Lattice:
     def set_tile(self, tile):
         """Tile shape of the fused pass: -1 automatic, 0 = 32x8, 1 = 64x4."""
         _check(self.lib.qm_lattice_set_tile(self.h, tile))
+
+    def autotune(self, force):
+        """Pick the fastest tile shape and z chunk of the fused FIRE pass for this
+        lattice and force (timed on the current state, which is unchanged)."""
+        t, z, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _check(self.lib.qm_lattice_autotune(self.h, force.h, ctypes.byref(t), ctypes.byref(z), ctypes.byref(ms)))
+        return t.value, z.value, ms.value
 
     def set_precision(self, precision):
         """Convert the device state to QM_FP64 or QM_FP32 (e.g. an FP64 polish)."""

@@ -200,3 +200,20 @@ def test_tile_shapes_agree(precision):
         m.minimize(f2, g2)
         traces.append(m.trace())
     np.testing.assert_allclose(traces[0][:, 0], traces[1][:, 0], rtol=1e-12 if precision == qmin.QM_FP64 else 1e-6)
+
+
+def test_autotune_leaves_state_and_results_unchanged():
+    o, g, f = build_pair((40, 24, 20), MULTI, 1, True)
+    m = qmin.FIRE(max_iter=10)
+    m.minimize(f, g)
+    o.fire(max_iter=10)
+    q0, v0 = g.q(), g.v()
+    tile, zchunk, ms = g.autotune(f)
+    assert tile in (0, 1) and zchunk in (0, 8, 16, 32, 64) and ms > 0
+    assert np.array_equal(g.q(), q0) and np.array_equal(g.v(), v0)
+    # the run continues on the tuned shape exactly as the oracle's uninterrupted one
+    m.set_max_iterations(25)
+    m.minimize(f, g)
+    o2, _, _ = build_pair((40, 24, 20), MULTI, 1, True)
+    it_ref, tr_ref = o2.fire(max_iter=25)
+    np.testing.assert_allclose(g.q(), o2.q(), rtol=0, atol=1e-11)
