@@ -43,6 +43,7 @@ typedef struct qm_fire qm_fire;
 typedef struct qm_adam qm_adam;
 typedef struct qm_nesterov qm_nesterov;
 typedef struct qm_lolbfgs qm_lolbfgs;
+typedef struct qm_ensemble qm_ensemble;
 
 const char* qm_last_error(void);
 int qm_device_count(int* count);
@@ -211,6 +212,20 @@ int qm_nesterov_set_trace(qm_nesterov* m, int capacity);
 int qm_nesterov_get_trace(const qm_nesterov* m, double* forceMax, int cap, int* n);
 int qm_lolbfgs_set_trace(qm_lolbfgs* m, int capacity);
 int qm_lolbfgs_get_trace(const qm_lolbfgs* m, double* forceMax, int cap, int* n);
+
+/* ---- ensembles: many lattices minimized together --------------------------
+ * No reference counterpart: the reference runs one lattice per process.  The
+ * lattices (any sizes, one precision) stay owned by the caller and must outlive
+ * the ensemble.  Each FIRE kernel is one launch over every lattice, every
+ * lattice keeps its own FIRE scalars and stops on its own; batches of
+ * graphIterations iterations are captured once in a hipGraph and replayed. */
+int qm_ensemble_create(qm_lattice* const* lattices, int count, qm_ensemble** out);
+int qm_ensemble_destroy(qm_ensemble* e);
+/* energyMinimizerFIRE::minimize on every lattice (fresh FIRE scalars from params'
+ * parameters, cutoff and iteration cap); per-lattice iterations and forceMax out
+ * (arrays of count, may be NULL) */
+int qm_ensemble_fire_minimize(qm_ensemble* e, const qm_fire* params, qm_ldg* f, int graphIterations,
+                              int* iterations, double* forceMax);
 
 /* ---- kernel timing (HIP events on the library's stream) -------------------
  * When enabled, every launch of the two FIRE kernels is bracketed by events;

@@ -173,8 +173,11 @@ __device__ __forceinline__ void st_out(R* p, R v) {
     __builtin_nontemporal_store(v, p);
 }
 
+// One work-group's share of the fused pass: the column of tile `logical` (x
+// tile fastest, then y tile, then z chunk) of the lattice described by A; its
+// partial sums go to slot `slot`.
 template <typename R, int MODE, int TERMS, int TILE>
-__global__ void __launch_bounds__(kThreads) march_kernel(MarchArgs<R> A) {
+__device__ __forceinline__ void march_body(const MarchArgs<R>& A, const int logical, const int slot) {
     using T = Tile<kTileTX[TILE], kTileTY[TILE]>;
     constexpr int kTX = T::TX, kTY = T::TY, kHX = T::HX, kHY = T::HY, kHP = T::HP, kHalo = T::Halo;
     __shared__ R sQ[3][5][kHP];
@@ -198,12 +201,6 @@ __global__ void __launch_bounds__(kThreads) march_kernel(MarchArgs<R> A) {
         cf = (reset || identity) ? R(0) : R(ctrl->cForce);
     }
 
-    // XCD-aware tile order: consecutive work-group ids go round-robin over the
-    // 8 XCDs; give each XCD a contiguous run of tiles so that tiles sharing
-    // halos sit in the same L2.
-    const int nb = A.nblocks, b = blockIdx.x;
-    const int qb = nb >> 3, rb = nb & 7, xcd = b & 7;
-    const int logical = xcd * qb + min(xcd, rb) + (b >> 3);
     const int txi = logical % A.nbx;
     const int rest = logical / A.nbx;
     const int tyi = rest % A.nby;
@@ -348,11 +345,35 @@ __global__ void __launch_bounds__(kThreads) march_kernel(MarchArgs<R> A) {
     if (MODE == kFire) {
         block_sum3<kThreads>(pF, pP, pV, sRed);
         if (tid == 0) {
-            A.partials[3 * (long long)b + 0] = pF;
-            A.partials[3 * (long long)b + 1] = pP;
-            A.partials[3 * (long long)b + 2] = pV;
+            A.partials[3 * (long long)slot + 0] = pF;
+            A.partials[3 * (long long)slot + 1] = pP;
+            A.partials[3 * (long long)slot + 2] = pV;
         }
     }
+}
+
+// XCD-aware work-group order: consecutive hardware ids go round-robin over the
+// 8 XCDs; each XCD gets a contiguous run of logical tiles, so that tiles sharing
+// halos share an L2.
+__device__ __forceinline__ int xcd_logical(int b, int nb) {
+    const int qb = nb >> 3, rb = nb & 7, xcd = b & 7;
+    return xcd * qb + min(xcd, rb) + (b >> 3);
+}
+
+template <typename R, int MODE, int TERMS, int TILE>
+__global__ void __launch_bounds__(kThreads) march_kernel(MarchArgs<R> A) {
+    march_body<R, MODE, TERMS, TILE>(A, xcd_logical(blockIdx.x, A.nblocks), blockIdx.x);
+}
+
+// The fused pass over many lattices in one launch: map[b] = {lattice, logical
+// tile} for hardware work-group b, laid out by the host in XCD-aware order.
+template <typename R, int MODE, int TERMS, int TILE>
+__global__ void __launch_bounds__(kThreads) march_batch_kernel(const MarchArgs<R>* __restrict__ descs,
+                                                               const int2* __restrict__ map) {
+    // uniform over the work-group: read once, keep in scalar registers
+    const int lat = __builtin_amdgcn_readfirstlane(map[blockIdx.x].x);
+    const int logical = __builtin_amdgcn_readfirstlane(map[blockIdx.x].y);
+    march_body<R, MODE, TERMS, TILE>(descs[lat], logical, logical);
 }
 
 // ---------------------------------------------------------------- boundary pass
@@ -414,8 +435,9 @@ __device__ void fire_scalar_step(Ctrl* ctrl, const FireConst& fp, double fnorm, 
     ctrl->done = !(it < ctrl->maxIter && forceMax > fp.fc);
 }
 
+// Work-group `blk` of the boundary pass of the lattice described by A.
 template <typename R, int MODE, int TERMS>
-__global__ void __launch_bounds__(kBoundaryThreads) boundary_kernel(BoundaryArgs<R> A) {
+__device__ __forceinline__ void boundary_body(const BoundaryArgs<R>& A, const int blk) {
     __shared__ double sRed[kBoundaryThreads / 64][3];
     Ctrl* ctrl = A.ctrl;
     if ((MODE == kFire || A.gate) && ctrl->done) return;
@@ -426,7 +448,7 @@ __global__ void __launch_bounds__(kBoundaryThreads) boundary_kernel(BoundaryArgs
     const int lx = A.lx, ly = A.ly, lz = A.lz;
     double pF = 0, pP = 0, pV = 0;
 
-    const long long k = (long long)blockIdx.x * kBoundaryThreads + threadIdx.x;
+    const long long k = (long long)blk * kBoundaryThreads + threadIdx.x;
     if (k < A.nsites) {
         const long long i = A.sites[k];
         const int self = A.types[i];
@@ -569,11 +591,24 @@ __global__ void __launch_bounds__(kBoundaryThreads) boundary_kernel(BoundaryArgs
     if (MODE != kFire) return;
     block_sum3<kBoundaryThreads>(pF, pP, pV, sRed);
     if (threadIdx.x == 0) {
-        const long long slot = A.marchBlocks + blockIdx.x;
+        const long long slot = A.marchBlocks + blk;
         A.partials[3 * slot + 0] = pF;
         A.partials[3 * slot + 1] = pP;
         A.partials[3 * slot + 2] = pV;
     }
+}
+
+template <typename R, int MODE, int TERMS>
+__global__ void __launch_bounds__(kBoundaryThreads) boundary_kernel(BoundaryArgs<R> A) {
+    boundary_body<R, MODE, TERMS>(A, blockIdx.x);
+}
+
+template <typename R, int MODE, int TERMS>
+__global__ void __launch_bounds__(kBoundaryThreads) boundary_batch_kernel(
+    const BoundaryArgs<R>* __restrict__ descs, const int2* __restrict__ map) {
+    const int lat = __builtin_amdgcn_readfirstlane(map[blockIdx.x].x);
+    const int blk = __builtin_amdgcn_readfirstlane(map[blockIdx.x].y);
+    boundary_body<R, MODE, TERMS>(descs[lat], blk);
 }
 
 // FIRE scalar step after both lattice passes of an iteration: one work-group
@@ -581,8 +616,8 @@ __global__ void __launch_bounds__(kBoundaryThreads) boundary_kernel(BoundaryArgs
 // blocks) and advances the FIRE scalars.  A kernel of its own: the launch
 // boundary makes the partials visible without device-wide fences, which cost
 // a write-back of each XCD's L2 right after the lattice pass has filled it.
-__global__ void __launch_bounds__(kReduceThreads) fire_finalize_kernel(Ctrl* ctrl, FireConst fp, const double* part,
-                                                                      long long total, double* trace) {
+__device__ __forceinline__ void fire_finalize_body(Ctrl* ctrl, const FireConst& fp, const double* part,
+                                                   long long total, double* trace) {
     __shared__ double sRed[kReduceThreads / 64][3];
     if (ctrl->done) return;
     double s0 = 0, s1 = 0, s2 = 0;
@@ -593,6 +628,26 @@ __global__ void __launch_bounds__(kReduceThreads) fire_finalize_kernel(Ctrl* ctr
     }
     block_sum3<kReduceThreads>(s0, s1, s2, sRed);
     if (threadIdx.x == 0) fire_scalar_step(ctrl, fp, s0, s1, s2, trace);
+}
+
+__global__ void __launch_bounds__(kReduceThreads) fire_finalize_kernel(Ctrl* ctrl, FireConst fp, const double* part,
+                                                                      long long total, double* trace) {
+    fire_finalize_body(ctrl, fp, part, total, trace);
+}
+
+// One work-group per lattice of an ensemble; each also copies its lattice's
+// progress into a compact status row for the host.
+__global__ void __launch_bounds__(kReduceThreads) fire_finalize_batch_kernel(const FinalizeDesc* descs,
+                                                                            EnsembleStatus* status) {
+    const FinalizeDesc& d = descs[blockIdx.x];
+    fire_finalize_body(d.ctrl, d.fire, d.partials, d.total, nullptr);
+    if (threadIdx.x == 0) {
+        EnsembleStatus st;
+        st.done = d.ctrl->done;
+        st.iterations = d.ctrl->iterations;
+        st.forceMax = d.ctrl->forceMax;
+        status[blockIdx.x] = st;
+    }
 }
 
 // ---------------------------------------------------------------- other passes
@@ -752,6 +807,52 @@ static void launch_boundary_t(const BoundaryArgs<R>& a, int blocks, hipStream_t 
     hipLaunchKernelGGL((boundary_kernel<R, MODE, TERMS>), dim3(blocks), dim3(kBoundaryThreads), 0, s, a);
 }
 
+template <typename R, int MODE, int TERMS>
+static const void* march_batch_fn_tile(int tile) {
+    return tile == kTile64x4 ? reinterpret_cast<const void*>(&march_batch_kernel<R, MODE, TERMS, kTile64x4>)
+                             : reinterpret_cast<const void*>(&march_batch_kernel<R, MODE, TERMS, kTile32x8>);
+}
+
+template <typename R, int MODE>
+static const void* march_batch_fn_terms(int terms, int tile) {
+    switch (terms) {
+        case kOneConstant: return march_batch_fn_tile<R, MODE, kOneConstant>(tile);
+        case kMultiLinear: return march_batch_fn_tile<R, MODE, kMultiLinear>(tile);
+        case kMultiL4: return march_batch_fn_tile<R, MODE, kMultiL4>(tile);
+        default: return march_batch_fn_tile<R, MODE, kMultiL6>(tile);
+    }
+}
+
+template <typename R>
+hipError_t launch_march_batch(const MarchArgs<R>* descs, const int2* map, int blocks, int mode, int terms, int tile,
+                              hipStream_t s) {
+    const void* fn = mode == kFire ? march_batch_fn_terms<R, kFire>(terms, tile)
+                                   : march_batch_fn_terms<R, kForce>(terms, tile);
+    void* args[] = {&descs, &map};
+    return hipLaunchKernel(fn, dim3(blocks), dim3(kThreads), args, 0, s);
+}
+
+template <typename R>
+hipError_t launch_boundary_batch(const BoundaryArgs<R>* descs, const int2* map, int blocks, int mode, int terms,
+                                 hipStream_t s) {
+    const void* fn;
+    if (mode == kFire)
+        fn = terms == kOneConstant ? reinterpret_cast<const void*>(&boundary_batch_kernel<R, kFire, kOneConstant>)
+           : terms == kMultiL6     ? reinterpret_cast<const void*>(&boundary_batch_kernel<R, kFire, kMultiL6>)
+                                   : reinterpret_cast<const void*>(&boundary_batch_kernel<R, kFire, kMultiLinear>);
+    else
+        fn = terms == kOneConstant ? reinterpret_cast<const void*>(&boundary_batch_kernel<R, kForce, kOneConstant>)
+           : terms == kMultiL6     ? reinterpret_cast<const void*>(&boundary_batch_kernel<R, kForce, kMultiL6>)
+                                   : reinterpret_cast<const void*>(&boundary_batch_kernel<R, kForce, kMultiLinear>);
+    void* args[] = {&descs, &map};
+    return hipLaunchKernel(fn, dim3(blocks), dim3(kBoundaryThreads), args, 0, s);
+}
+
+hipError_t launch_fire_finalize_batch(const FinalizeDesc* descs, EnsembleStatus* status, int count, hipStream_t s) {
+    hipLaunchKernelGGL(fire_finalize_batch_kernel, dim3(count), dim3(kReduceThreads), 0, s, descs, status);
+    return hipGetLastError();
+}
+
 template <typename R>
 hipError_t launch_march(const MarchArgs<R>& a, int mode, int terms, int tile, hipStream_t s) {
     void* args[] = {const_cast<MarchArgs<R>*>(&a)};
@@ -827,7 +928,9 @@ void launch_defects(const R* q, const int8_t* types, long long n, int kind, doub
     template int launch_energy<R>(const R*, const int8_t*, const DevObject*, int, int, int, const EnergyConst&,   \
                                   double*, hipStream_t);                                                        \
     template void launch_defects<R>(const R*, const int8_t*, long long, int, double*, hipStream_t);             \
-    template int march_blocks_per_cu<R>(int, int, int);
+    template int march_blocks_per_cu<R>(int, int, int);                                                          \
+    template hipError_t launch_march_batch<R>(const MarchArgs<R>*, const int2*, int, int, int, int, hipStream_t);  \
+    template hipError_t launch_boundary_batch<R>(const BoundaryArgs<R>*, const int2*, int, int, int, hipStream_t);
 
 QMIN_INSTANTIATE(double)
 QMIN_INSTANTIATE(float)

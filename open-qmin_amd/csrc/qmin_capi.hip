@@ -636,7 +636,7 @@ static ForceConst force_const(const qm_ldg* f) {
     return fc;
 }
 
-static int terms_of(const qm_ldg* f) {
+int qmin::terms_of(const qm_ldg* f) {
     if (!f->multi) return kOneConstant;
     if (f->L6 != 0) return kMultiL6;
     return f->L4 != 0 ? kMultiL4 : kMultiLinear;
@@ -675,17 +675,17 @@ static int upload_weights(qm_lattice* L, const qm_ldg* f) {
 
 // Tile shape of the fused pass: the one set on the lattice, else the measured
 // better default per precision (32 x 8 for FP64, 64 x 4 for FP32).
-static int active_tile(const qm_lattice* L) {
+int qmin::active_tile(const qm_lattice* L) {
     if (L->tileOverride >= 0) return L->tileOverride;
     return L->precision == QM_FP64 ? kTile32x8 : kTile64x4;
 }
 
-static long long tile_columns(const qm_lattice* L, int tile) {
+long long qmin::tile_columns(const qm_lattice* L, int tile) {
     return (long long)((L->lx + kTileTX[tile] - 1) / kTileTX[tile]) * ((L->ly + kTileTY[tile] - 1) / kTileTY[tile]);
 }
 
 template <typename R>
-static void fill_args(qm_lattice* L, const qm_ldg* f, MarchArgs<R>& ma, BoundaryArgs<R>& ba, const FireConst& fire) {
+void qmin::fill_args(qm_lattice* L, const qm_ldg* f, MarchArgs<R>& ma, BoundaryArgs<R>& ba, const FireConst& fire) {
     ma.set[0] = state_set<R>(L, 0);
     ma.set[1] = state_set<R>(L, 1);
     ma.cls = L->dClass;
@@ -720,6 +720,11 @@ static void fill_args(qm_lattice* L, const qm_ldg* f, MarchArgs<R>& ma, Boundary
     ba.partials = L->dPartials;
     ba.trace = L->dTrace;
 }
+
+template void qmin::fill_args<double>(qm_lattice*, const qm_ldg*, MarchArgs<double>&, BoundaryArgs<double>&,
+                                      const FireConst&);
+template void qmin::fill_args<float>(qm_lattice*, const qm_ldg*, MarchArgs<float>&, BoundaryArgs<float>&,
+                                     const FireConst&);
 
 // z chunk of the fused pass: as long as possible while the grid still fills
 // every CU to its occupancy, so that the whole grid is resident at once, all
@@ -1022,7 +1027,7 @@ int qm_fire_get_state(const qm_fire* m, qm_fire_state* s) {
     return QM_OK;
 }
 
-static FireConst fire_const(const qm_fire* m, const qm_lattice* L) {
+FireConst qmin::fire_const(const qm_fire* m, const qm_lattice* L) {
     FireConst fp{};
     fp.alphaStart = m->alphaStart;
     fp.dtMax = m->dtMax;

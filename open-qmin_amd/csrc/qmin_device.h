@@ -122,4 +122,18 @@ struct BoundaryArgs {
     double* trace;          // [traceCap][6]
 };
 
+// Per-lattice arguments of the batched FIRE scalar step.
+struct FinalizeDesc {
+    Ctrl* ctrl;
+    FireConst fire;
+    const double* partials;
+    long long total;
+};
+
+struct EnsembleStatus {
+    int done;
+    int iterations;
+    double forceMax;
+};
+
 }  // namespace qmin

@@ -11,6 +11,9 @@
 using qmin::Ctrl;
 using qmin::DevObject;
 using qmin::StateSet;
+using qmin::MarchArgs;
+using qmin::BoundaryArgs;
+using qmin::FireConst;
 
 struct HostObject {
     int type;
@@ -81,6 +84,16 @@ int materialize(qm_lattice* L);
 int push_ctrl(qm_lattice* L);
 int ensure_partials(qm_lattice* L, size_t count);
 int64_t n_active(const qm_lattice* L);
+// stencil variant of a force computer (kOneConstant .. kMultiL6)
+int terms_of(const qm_ldg* f);
+// tile shape of the fused pass for this lattice, and its tile columns
+int active_tile(const qm_lattice* L);
+long long tile_columns(const qm_lattice* L, int tile);
+// kernel arguments of one lattice's passes (z chunk taken from L->zchunk)
+template <typename R>
+void fill_args(qm_lattice* L, const qm_ldg* f, MarchArgs<R>& ma, BoundaryArgs<R>& ba, const FireConst& fire);
+FireConst fire_const(const qm_fire* m, const qm_lattice* L);
+int boundary_blocks(long long nsites);
 template <typename R>
 inline StateSet<R> state_set(const qm_lattice* L, int s) {
     R* base = static_cast<R*>(L->dSet[s]);

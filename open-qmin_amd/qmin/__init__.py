@@ -43,6 +43,9 @@ _API = {
     "qm_lattice_dims": [_VP] + [ctypes.POINTER(ctypes.c_int)] * 4,
     "qm_lattice_set_zchunk": [_VP, _i],
     "qm_lattice_set_tile": [_VP, _i],
+    "qm_ensemble_create": [ctypes.POINTER(_VP), _i, ctypes.POINTER(_VP)],
+    "qm_ensemble_destroy": [_VP],
+    "qm_ensemble_fire_minimize": [_VP, _VP, _VP, _i, ctypes.POINTER(ctypes.c_int), _D],
     "qm_lattice_autotune": [_VP, _VP, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), _D],
     "qm_lattice_set_precision": [_VP, _i],
     "qm_lattice_set_field": [_VP, _i, _D],
@@ -439,6 +442,36 @@ class LoLBFGS(_SimpleMinimizer):
         super().__init__()
         _check(self.lib.qm_lolbfgs_set_parameters(self.h, m, dt, c, fc, tau, int(scheduled)))
         self.set_max_iterations(max_iter)
+
+
+class Ensemble:
+    """Many lattices (one precision) minimized together: one launch per kernel
+    over all of them, hipGraph-replayed batches of iterations."""
+
+    def __init__(self, lattices):
+        self.lib = library()
+        self.lattices = list(lattices)      # keep them alive as long as the ensemble
+        arr = (_VP * len(self.lattices))(*[l.h for l in self.lattices])
+        h = _VP()
+        _check(self.lib.qm_ensemble_create(arr, len(self.lattices), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.qm_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def fire_minimize(self, fire, force, graph_iterations=16):
+        """FIRE with fire's parameters on every lattice; returns per-lattice
+        (iterations, forceMax) arrays."""
+        n = len(self.lattices)
+        it = (ctypes.c_int * n)()
+        fm = np.zeros(n)
+        _check(self.lib.qm_ensemble_fire_minimize(self.h, fire.h, force.h, graph_iterations, it, _dp(fm)))
+        return np.array(list(it)), fm
 
 
 def timing_enable(on=True):
