@@ -113,6 +113,13 @@ int qm_lattice_save_state(qm_lattice* lat, const char* path);
 int qm_lattice_load_state(qm_lattice* lat, const char* path);
 /* copy the device state Q of one lattice into another of the same size */
 int qm_lattice_copy_q(qm_lattice* dst, const qm_lattice* src);
+/* spatially varying H field of the lattice, AoS double[N][3] (NULL removes it);
+ * the reference keeps it in landauDeGennesLC::spatiallyVaryingField
+ * (landauDeGennesLC.h:93, .cpp:81-123) */
+int qm_lattice_set_hfield(qm_lattice* lat, const double* h3);
+/* the reference's field file ("x y z Hx Hy Hz" per line, landauDeGennesLC.cpp:69-123);
+ * path is the full file name (the reference appends _x%iy%iz%i.txt per rank) */
+int qm_lattice_load_hfield(qm_lattice* lat, const char* path);
 
 /* ---- force computer: landauDeGennesLC (src/forces/landauDeGennesLC.h:12-173) */
 int qm_ldg_create(qm_ldg** out);
@@ -131,6 +138,11 @@ int qm_ldg_set_hfield(qm_ldg* f, int on, double Hx, double Hy, double Hz, double
  * QM_SEMANTICS_REF_CPU: the reference CPU path's two passes, which correct bulk
  * sites twice (multirankSimulation.cpp:462-466). */
 int qm_ldg_set_semantics(qm_ldg* f, int semantics);
+/* setSpatiallyVaryingField's constants (landauDeGennesLC.cpp:81-87): on = 1 adds
+ * the lattice's varying H field (computeSpatiallyVaryingFieldCPU,
+ * landauDeGennesLCOtherForces.cpp:254-283; energy landauDeGennesLC.cpp:256-264);
+ * chi, mu0, deltaChi are shared with the uniform H field, as in the reference */
+int qm_ldg_set_varying_hfield(qm_ldg* f, int on, double chi, double mu0, double dchi);
 /* force::computeForces (src/forces/baseForce.h:15-77): F = forces(Q) into the lattice */
 int qm_ldg_compute_forces(qm_ldg* f, qm_lattice* lat);
 /* landauDeGennesLC::computeEnergy (landauDeGennesLC.cpp:204-353): total energy and

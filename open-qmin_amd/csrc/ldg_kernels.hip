@@ -155,12 +155,17 @@ __device__ __forceinline__ void fast_raw_force(const ForceConst& fc, ConstPtr<R>
     for (int c = 0; c < 5; ++c) F[c] -= spatial[c];
 }
 
+// fv: the site's spatially varying field force, or null
 template <typename R>
-__device__ __forceinline__ void finish_bulk(const ForceConst& fc, R* F) {
+__device__ __forceinline__ void finish_bulk(const ForceConst& fc, R* F, const R* fv) {
     if (fc.semantics == 0) metric(F);
     if (fc.hasField) {
 #pragma unroll
         for (int c = 0; c < 5; ++c) F[c] += R(fc.field[c]);
+    }
+    if (fv) {
+#pragma unroll
+        for (int c = 0; c < 5; ++c) F[c] += fv[c];
     }
     metric(F);
 }
@@ -306,7 +311,13 @@ __device__ __forceinline__ void march_body(const MarchArgs<R>& A, const int logi
                     asm volatile("" : "+s"(wa));
                     fast_raw_force<R, TERMS, kHX, kHP>(A.fc, reinterpret_cast<ConstPtr<R>>(wa), sQ, sm, sc, sp, hOwn,
                                                        q, F);
-                    finish_bulk(A.fc, F);
+                    if (A.hvar) {
+                        R fv[5];
+                        varying_field_force(A.hvar, n, i, A.hvarP, fv);
+                        finish_bulk(A.fc, F, fv);
+                    } else {
+                        finish_bulk<R>(A.fc, F, nullptr);
+                    }
                 } else {
 #pragma unroll
                     for (int c = 0; c < 5; ++c) F[c] = R(0);
@@ -558,9 +569,11 @@ __device__ __forceinline__ void boundary_body(const BoundaryArgs<R>& A, const in
         }
 #pragma unroll
         for (int c = 0; c < 5; ++c) F[c] -= spatial[c];
+        R fv[5];
+        if (A.hvar) varying_field_force(A.hvar, n, i, A.hvarP, fv);
         if (self == 0) {
             // a bulk site with a one-sided neighbour derivative: bulk semantics
-            finish_bulk(A.fc, F);
+            finish_bulk(A.fc, F, A.hvar ? fv : nullptr);
         } else {
 #pragma unroll
             for (int d = 0; d < 6; ++d)
@@ -568,6 +581,10 @@ __device__ __forceinline__ void boundary_body(const BoundaryArgs<R>& A, const in
             if (A.fc.hasField) {
 #pragma unroll
                 for (int c = 0; c < 5; ++c) F[c] += R(A.fc.field[c]);
+            }
+            if (A.hvar) {
+#pragma unroll
+                for (int c = 0; c < 5; ++c) F[c] += fv[c];
             }
             metric(F);
         }
@@ -692,6 +709,10 @@ __global__ void __launch_bounds__(kReduceThreads) energy_kernel(const R* q, cons
         acc[0] += double(R(ec.a) * t2 + R(ec.b) * trQ3(qc) + R(ec.c) * t2 * t2);
         if (ec.eOn) acc[3] += double(field_energy(ec.E, ec.eps, ec.deps, ec.eps0, qc));
         if (ec.hOn) acc[4] += double(field_energy(ec.H, ec.chi, ec.dchi, ec.mu0, qc));
+        if (ec.hvar) {
+            const double h3[3] = {ec.hvar[i], ec.hvar[n + i], ec.hvar[2 * n + i]};
+            acc[4] += double(field_energy(h3, ec.chi, ec.dchi, ec.mu0, qc));
+        }
         R dv[3][5];
 #pragma unroll
         for (int ax = 0; ax < 3; ++ax)

@@ -254,6 +254,19 @@ __device__ __forceinline__ R field_energy(const double* Fv, double iso, double a
     return R(-0.5) * R(perm) * (R(iso) * ff + R(anis) * fqf);
 }
 
+// Force of a spatially varying H field at site i (planes hx, hy, hz of n):
+// minus the reference's fieldForce, as computeSpatiallyVaryingFieldCPU
+// subtracts it (landauDeGennesLCOtherForces.cpp:254-283); P = deltaChi mu0.
+template <typename R>
+__device__ __forceinline__ void varying_field_force(const double* hvar, long long n, long long i, double P, R* out) {
+    const double hx = hvar[i], hy = hvar[n + i], hz = hvar[2 * n + i];
+    out[0] = R(-(-0.5 * P * (hx * hx - hz * hz)));
+    out[1] = R(-(-P * hx * hy));
+    out[2] = R(-(-P * hx * hz));
+    out[3] = R(-(-0.5 * P * (hy * hy - hz * hz)));
+    out[4] = R(-(-P * hy * hz));
+}
+
 // QT inner product (inc/functions.h:140-147): sum a_i b_i + a_0 b_3
 template <typename R>
 __device__ __forceinline__ double qt_dot(const R* a, const R* b) {

@@ -235,6 +235,7 @@ int qm_lattice_destroy(qm_lattice* L) {
     (void)hipFree(L->dCtrl);
     if (L->dPartials) (void)hipFree(L->dPartials);
     if (L->dTrace) (void)hipFree(L->dTrace);
+    if (L->dHField) (void)hipFree(L->dHField);
     (void)hipStreamDestroy(L->stream);
     delete L;
     return QM_OK;
@@ -279,6 +280,42 @@ int qm_lattice_set_precision(qm_lattice* L, int precision) {
     L->precision = precision;
     L->realSize = realSize;
     return QM_OK;
+}
+
+// Spatially varying H field of the lattice, AoS [N][3] (NULL removes it).
+int qm_lattice_set_hfield(qm_lattice* L, const double* h3) {
+    if (!L) return fail(QM_EINVAL, "null lattice");
+    if (!h3) {
+        if (L->dHField) HIP_TRY(hipFree(L->dHField));
+        L->dHField = nullptr;
+        return QM_OK;
+    }
+    const long long n = L->n;
+    std::vector<double> planes(3 * (size_t)n);
+    for (long long i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) planes[(size_t)a * n + i] = h3[3 * i + a];
+    if (!L->dHField) HIP_TRY(hipMalloc(&L->dHField, 3 * (size_t)n * sizeof(double)));
+    HIP_TRY(hipMemcpy(L->dHField, planes.data(), 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+    return QM_OK;
+}
+
+// landauDeGennesLC::setSpatiallyVaryingField's file reading
+// (landauDeGennesLC.cpp:81-123): lines "x y z Hx Hy Hz", unlisted sites zero;
+// path is the full file name.
+int qm_lattice_load_hfield(qm_lattice* L, const char* path) {
+    if (!L || !path) return fail(QM_EINVAL, "bad argument");
+    FILE* fp = std::fopen(path, "r");
+    if (!fp) return fail(QM_EIO, std::string("cannot read ") + path);
+    std::vector<double> h3(3 * (size_t)L->n, 0.0);
+    double fx, fy, fz, hx, hy, hz;
+    while (std::fscanf(fp, "%lf %lf %lf %lf %lf %lf", &fx, &fy, &fz, &hx, &hy, &hz) == 6) {
+        const long long i = site_of(L, wrapc((int)fx, L->lx), wrapc((int)fy, L->ly), wrapc((int)fz, L->lz));
+        h3[3 * i + 0] = hx;
+        h3[3 * i + 1] = hy;
+        h3[3 * i + 2] = hz;
+    }
+    std::fclose(fp);
+    return qm_lattice_set_hfield(L, h3.data());
 }
 
 int qm_lattice_set_tile(qm_lattice* L, int tile) {
@@ -595,6 +632,16 @@ int qm_ldg_set_hfield(qm_ldg* f, int on, double Hx, double Hy, double Hz, double
     f->chi = chi; f->mu0 = mu0; f->dchi = dchi;
     return QM_OK;
 }
+int qm_ldg_set_varying_hfield(qm_ldg* f, int on, double chi, double mu0, double dchi) {
+    if (!f) return fail(QM_EINVAL, "null force");
+    f->hvarOn = on != 0;
+    // shared with the uniform H field, as in the reference
+    f->chi = chi;
+    f->mu0 = mu0;
+    f->dchi = dchi;
+    return QM_OK;
+}
+
 int qm_ldg_set_semantics(qm_ldg* f, int s) {
     if (!f || (s != QM_SEMANTICS_REF_CPU && s != QM_SEMANTICS_REF_GPU)) return fail(QM_EINVAL, "bad semantics");
     f->semantics = s;
@@ -719,6 +766,10 @@ void qmin::fill_args(qm_lattice* L, const qm_ldg* f, MarchArgs<R>& ma, BoundaryA
     ba.ctrl = L->dCtrl;
     ba.partials = L->dPartials;
     ba.trace = L->dTrace;
+    ma.hvar = f->hvarOn ? L->dHField : nullptr;
+    ma.hvarP = f->dchi * f->mu0;
+    ba.hvar = ma.hvar;
+    ba.hvarP = ma.hvarP;
 }
 
 template void qmin::fill_args<double>(qm_lattice*, const qm_ldg*, MarchArgs<double>&, BoundaryArgs<double>&,
@@ -868,6 +919,8 @@ int qm_lattice_autotune(qm_lattice* L, qm_ldg* f, int* tileOut, int* zchunkOut, 
 
 int qmin::prepare(qm_lattice* L, const qm_ldg* f) {
     if (!L || !f) return fail(QM_EINVAL, "null handle");
+    if (f->hvarOn && !L->dHField)
+        return fail(QM_ESTATE, "the force uses a spatially varying field but the lattice has none");
     if (int rc = refresh_types(L)) return rc;
     if (int rc = upload_weights(L, f)) return rc;
     // room for the smallest z chunk (4 planes) the fused pass may pick
@@ -921,6 +974,7 @@ int qm_ldg_compute_energy(qm_ldg* f, qm_lattice* L, double* total, double* comps
     ec.chi = f->chi;
     ec.mu0 = f->mu0;
     ec.dchi = f->dchi;
+    ec.hvar = f->hvarOn ? L->dHField : nullptr;
     if (int rc = ensure_partials(L, 5 * (size_t)kMaxEnergyBlocks)) return rc;
     int blocks;
     if (L->precision == QM_FP64)

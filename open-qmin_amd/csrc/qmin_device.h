@@ -81,6 +81,7 @@ struct EnergyConst {
     int eOn, hOn;
     double E[3], eps, eps0, deps;
     double H[3], chi, mu0, dchi;
+    const double* hvar;   // spatially varying H, planes hx, hy, hz of n (null: none)
 };
 
 template <typename R>
@@ -102,6 +103,8 @@ struct MarchArgs {
     Ctrl* ctrl;
     double* partials;   // [nblocks + boundary blocks][3]
     const R* W;         // composed bulk operator weights (device buffer, read with scalar loads)
+    const double* hvar; // spatially varying H, planes hx, hy, hz of n (null: none)
+    double hvarP;       // deltaChi mu0
 };
 
 template <typename R>
@@ -120,6 +123,8 @@ struct BoundaryArgs {
     Ctrl* ctrl;
     double* partials;
     double* trace;          // [traceCap][6]
+    const double* hvar;
+    double hvarP;
 };
 
 // Per-lattice arguments of the batched FIRE scalar step.

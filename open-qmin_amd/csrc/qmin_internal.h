@@ -44,6 +44,7 @@ struct qm_lattice {
     int zchunk = 16;          // z extent one work-group marches through
     int zchunkOverride = 0;   // fixed z chunk (0 = choose from occupancy)
     int tileOverride = -1;    // tile shape of the fused pass (-1 = by precision)
+    double* dHField = nullptr;  // spatially varying H: planes hx, hy, hz of n (FP64)
     int numCUs = 0;
 };
 
@@ -55,6 +56,7 @@ struct qm_ldg {
     int eOn = 0, hOn = 0;
     double E[3] = {0, 0, 0}, eps = 1, eps0 = 1, deps = 0.5;
     double H[3] = {0, 0, 0}, chi = 1, mu0 = 1, dchi = 0.5;
+    int hvarOn = 0;   // use the lattice's spatially varying H field
 };
 
 struct qm_fire {

@@ -4,7 +4,8 @@
 // Flags keep the reference's names and defaults (openQmin.cpp:59-115):
 //   -l/--boxL, --Lx/--Ly/--Lz, -i/--iterations, -f/--fTarget, -e/--deltaT,
 //   -a/-b/-c phase constants, --L1 --L2 --L3 --L4 --L6, --randomSeed,
-//   --boundaryFile, --saveFile, --initialConfigurationFile, --hField{X,Y,Z,Mu0,Chi,DeltaChi},
+//   --boundaryFile, --saveFile, --initialConfigurationFile, --spatiallyVaryingFieldFile,
+//   --hField{X,Y,Z,Mu0,Chi,DeltaChi},
 //   --eField{X,Y,Z,Epsilon0,Epsilon,DeltaEpsilon}, -z/--initializationSwitch (0 only), -v.
 // Additions of this build (no reference flag):
 //   --minimizer fire|adam|nesterov|lolbfgs   (default fire, as openQmin)
@@ -95,8 +96,6 @@ Args parse(int argc, char** argv) {
     if (a.has("precision") && a.v.at("precision") != "fp64" && a.v.at("precision") != "fp32" &&
         a.v.at("precision") != "mixed")
         throw std::runtime_error("--precision is fp64, fp32 or mixed");
-    if (a.has("spatiallyVaryingFieldFile") && a.v.at("spatiallyVaryingFieldFile") != "NONE")
-        throw std::runtime_error("spatially varying fields are not provided");
     return a;
 }
 
@@ -148,10 +147,12 @@ int main(int argc, char** argv) {
         if (E.x != 0 || E.y != 0 || E.z != 0)
             landauLCForce->setEField(E, a.d("eFieldEpsilon", 1), a.d("eFieldEpsilon0", 1),
                                      a.d("eFieldDeltaEpsilon", 0.5));
-        if (a.s("spatiallyVaryingFieldFile", "NONE") != "NONE")
-            throw std::runtime_error("spatially varying fields are not provided");
         landauLCForce->setModel(Configuration);
         sim->addForce(landauLCForce);
+        // openQmin.cpp:245-248
+        if (a.s("spatiallyVaryingFieldFile", "NONE") != "NONE")
+            landauLCForce->setSpatiallyVaryingField(a.s("spatiallyVaryingFieldFile", ""), a.d("hFieldChi", 1),
+                                                    a.d("hFieldMu0", 1), a.d("hFieldDeltaChi", 0.5));
         sim->setCPUOperation(a.has("cpuSemantics"));
 
         std::shared_ptr<updater> upd;

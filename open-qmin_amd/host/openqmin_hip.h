@@ -212,6 +212,13 @@ class landauDeGennesLC {
     void setHField(scalar3 H, scalar chi, scalar mu0, scalar deltaChi) {
         qmin_host::check(qm_ldg_set_hfield(f_, 1, H.x, H.y, H.z, chi, mu0, deltaChi), "setHField");
     }
+    // landauDeGennesLC::setSpatiallyVaryingField (landauDeGennesLC.cpp:81-123):
+    // reads fname_x0y0z0.txt (one rank) into the model's field
+    void setSpatiallyVaryingField(std::string fname, scalar chi, scalar mu0, scalar deltaChi) {
+        const std::string fn = fname + "_x0y0z0.txt";
+        qmin_host::check(qm_lattice_load_hfield(lattice(), fn.c_str()), "setSpatiallyVaryingField");
+        qmin_host::check(qm_ldg_set_varying_hfield(f_, 1, chi, mu0, deltaChi), "setSpatiallyVaryingField");
+    }
     // QM_SEMANTICS_REF_CPU or QM_SEMANTICS_REF_GPU
     void setSemantics(int s) { qmin_host::check(qm_ldg_set_semantics(f_, s), "setSemantics"); }
     void setModel(std::shared_ptr<cubicLattice> m) { model = m; }

@@ -43,6 +43,9 @@ _API = {
     "qm_lattice_dims": [_VP] + [ctypes.POINTER(ctypes.c_int)] * 4,
     "qm_lattice_set_zchunk": [_VP, _i],
     "qm_lattice_set_tile": [_VP, _i],
+    "qm_lattice_set_hfield": [_VP, _D],
+    "qm_lattice_load_hfield": [_VP, ctypes.c_char_p],
+    "qm_ldg_set_varying_hfield": [_VP, _i, _d, _d, _d],
     "qm_ensemble_create": [ctypes.POINTER(_VP), _i, ctypes.POINTER(_VP)],
     "qm_ensemble_destroy": [_VP],
     "qm_ensemble_fire_minimize": [_VP, _VP, _VP, _i, ctypes.POINTER(ctypes.c_int), _D],
@@ -169,6 +172,17 @@ class Lattice:
 
     def set_zchunk(self, zchunk):
         _check(self.lib.qm_lattice_set_zchunk(self.h, zchunk))
+
+    def set_hfield(self, h3):
+        """Spatially varying H field, [n][3] in site order (None removes it)."""
+        if h3 is None:
+            _check(self.lib.qm_lattice_set_hfield(self.h, None))
+            return
+        a = np.ascontiguousarray(h3, np.float64).reshape(self.n, 3)
+        _check(self.lib.qm_lattice_set_hfield(self.h, _dp(a)))
+
+    def load_hfield(self, path):
+        _check(self.lib.qm_lattice_load_hfield(self.h, str(path).encode()))
 
     def set_tile(self, tile):
         """Tile shape of the fused pass: -1 automatic, 0 = 32x8, 1 = 64x4."""
@@ -301,6 +315,10 @@ class LandauDeGennesLC:
 
     def set_hfield(self, H, chi=1.0, mu0=1.0, dchi=0.5, on=True):
         _check(self.lib.qm_ldg_set_hfield(self.h, int(on), H[0], H[1], H[2], chi, mu0, dchi))
+
+    def set_varying_hfield(self, chi=1.0, mu0=1.0, dchi=0.5, on=True):
+        """Add the lattice's spatially varying H field (Lattice.set_hfield / load_hfield)."""
+        _check(self.lib.qm_ldg_set_varying_hfield(self.h, int(on), chi, mu0, dchi))
 
     def set_semantics(self, s):
         _check(self.lib.qm_ldg_set_semantics(self.h, s))

@@ -78,6 +78,7 @@ def oracle_lib():
         _sig(lib, "orc_set_params", None, [_VP] + [_d] * 8 + [_i, _i])
         _sig(lib, "orc_set_efield", None, [_VP, _i] + [_d] * 6)
         _sig(lib, "orc_set_hfield", None, [_VP, _i] + [_d] * 6)
+        _sig(lib, "orc_set_hfield_varying", None, [_VP, _i, _D] + [_d] * 3)
         _sig(lib, "orc_load_tables", None, [_i, _I, _I, _I, _D, _I, _I])
         _sig(lib, "orc_forces", None, [_VP, _D, _D])
         _sig(lib, "orc_first_derivatives", None, [_VP, _D, _D])
@@ -240,6 +241,12 @@ class OracleSystem:
 
     def set_hfield(self, H, chi=1.0, mu0=1.0, dchi=0.5, on=True):
         self.lib.orc_set_hfield(self.h, int(on), H[0], H[1], H[2], chi, mu0, dchi)
+
+    def set_hfield_varying(self, H3, chi=1.0, mu0=1.0, dchi=0.5, on=True):
+        """per-site H field, array [n][3] in site order"""
+        h = np.ascontiguousarray(H3, np.float64).reshape(self.n, 3)
+        self._hvar = h
+        self.lib.orc_set_hfield_varying(self.h, int(on), dptr(h), chi, mu0, dchi)
 
     def forces(self, q=None):
         q = self.q() if q is None else np.ascontiguousarray(q, np.float64)

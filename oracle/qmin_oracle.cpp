@@ -37,6 +37,8 @@ struct orc_system {
     int eOn = 0, hOn = 0;
     double E[3] = {0, 0, 0}, eps = 1, eps0 = 1, deps = 0;
     double H[3] = {0, 0, 0}, chi = 1, mu0 = 1, dchi = 0;
+    int hvarOn = 0;
+    std::vector<double> hvar;   // spatially varying H, [n][3]
     int nthreads = 1;
     std::vector<double> deriv;  // forceCalculationAssist, [n][15]
 };
@@ -419,7 +421,10 @@ void anchoringSum(const orc_system* s, int64_t i, const double* q, double* f) {
     }
 }
 
-void fieldSum(const orc_system* s, double* f) {
+// uniform E, uniform H, then the spatially varying H of site i
+// (landauDeGennesLC.cpp:190-200; computeSpatiallyVaryingFieldCPU,
+// landauDeGennesLCOtherForces.cpp:254-283)
+void fieldSum(const orc_system* s, int64_t i, double* f) {
     double tmp[5];
     if (s->eOn) {
         fieldForce(s->E, s->deps, s->eps0, tmp);
@@ -427,6 +432,10 @@ void fieldSum(const orc_system* s, double* f) {
     }
     if (s->hOn) {
         fieldForce(s->H, s->dchi, s->mu0, tmp);
+        for (int c = 0; c < 5; ++c) f[c] -= tmp[c];
+    }
+    if (s->hvarOn) {
+        fieldForce(&s->hvar[3 * i], s->dchi, s->mu0, tmp);
         for (int c = 0; c < 5; ++c) f[c] -= tmp[c];
     }
 }
@@ -453,19 +462,19 @@ void computeForces(orc_system* s, const double* q, double* f) {
             if (t == 0) {
                 for (int c = 0; c < 5; ++c) fi[c] = raw[c];
                 metric(fi);
-                fieldSum(s, fi);
+                fieldSum(s, i, fi);
                 metric(fi);
             } else {
                 // pass 1 left zero (metric of zero is zero); pass 2 adds
                 for (int c = 0; c < 5; ++c) fi[c] += raw[c];
                 anchoringSum(s, i, q, fi);
-                fieldSum(s, fi);
+                fieldSum(s, i, fi);
                 metric(fi);
             }
         } else {
             for (int c = 0; c < 5; ++c) fi[c] = raw[c];
             if (t < 0) anchoringSum(s, i, q, fi);
-            fieldSum(s, fi);
+            fieldSum(s, i, fi);
             metric(fi);
         }
     }
@@ -682,6 +691,16 @@ void orc_set_efield(orc_system* s, int on, double Ex, double Ey, double Ez, doub
     s->eps = eps; s->eps0 = eps0; s->deps = deps;
 }
 
+// landauDeGennesLC::setSpatiallyVaryingField (landauDeGennesLC.cpp:81-123): the
+// per-site field, with chi, mu0, deltaChi shared with the uniform H field
+void orc_set_hfield_varying(orc_system* s, int on, const double* h3, double chi, double mu0, double dchi) {
+    s->hvarOn = on;
+    if (on) s->hvar.assign(h3, h3 + 3 * s->n);
+    s->chi = chi;
+    s->mu0 = mu0;
+    s->dchi = dchi;
+}
+
 void orc_set_hfield(orc_system* s, int on, double Hx, double Hy, double Hz, double chi, double mu0, double dchi) {
     s->hOn = on;
     s->H[0] = Hx; s->H[1] = Hy; s->H[2] = Hz;
@@ -722,6 +741,7 @@ double orc_energy(orc_system* s, const double* q, double* comps) {
         phase += a * t2 + b * trQ3(qc) + c * t2 * t2;
         if (s->eOn) eF += fieldEnergy(s->E, s->eps, s->deps, s->eps0, qc);
         if (s->hOn) hF += fieldEnergy(s->H, s->chi, s->dchi, s->mu0, qc);
+        if (s->hvarOn) hF += fieldEnergy(&s->hvar[3 * i], s->chi, s->dchi, s->mu0, qc);
         const int32_t* nb = &s->nbr[6 * i];
         double d[3][5];
         for (int ax = 0; ax < 3; ++ax)

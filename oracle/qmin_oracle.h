@@ -47,6 +47,7 @@ void orc_set_params(orc_system* s, double A, double B, double C, double L1, doub
                     double L4, double L6, int multiConstant, int semantics);
 void orc_set_efield(orc_system* s, int on, double Ex, double Ey, double Ez, double eps, double eps0, double deps);
 void orc_set_hfield(orc_system* s, int on, double Hx, double Hy, double Hz, double chi, double mu0, double dchi);
+void orc_set_hfield_varying(orc_system* s, int on, const double* h3, double chi, double mu0, double dchi);
 // stencil tables: n entries, each (term slot 0..3 = L2,L3,L4,L6; case slot 0..27;
 // component; coefficient; input a; input b or -1)
 void orc_load_tables(int n, const int32_t* term, const int32_t* cslot, const int32_t* comp, const double* coef,

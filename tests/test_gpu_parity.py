@@ -217,3 +217,52 @@ def test_autotune_leaves_state_and_results_unchanged():
     o2, _, _ = build_pair((40, 24, 20), MULTI, 1, True)
     it_ref, tr_ref = o2.fire(max_iter=25)
     np.testing.assert_allclose(g.q(), o2.q(), rtol=0, atol=1e-11)
+
+
+def _random_field(n, seed):
+    rng = np.random.default_rng(seed)
+    return rng.normal(0.0, 0.4, size=(n, 3))
+
+
+@pytest.mark.parametrize("semantics,params,precision", [(0, ALL, qmin.QM_FP64), (1, MULTI, qmin.QM_FP64),
+                                                        (1, ONE, qmin.QM_FP32)])
+def test_spatially_varying_field_forces_energy(semantics, params, precision):
+    shape = (26, 18, 14)
+    o, g, f = build_pair(shape, params, semantics, True, precision=precision)
+    h = _random_field(g.n, 5)
+    o.set_hfield_varying(h, 1.0, 1.0, 0.7)
+    g.set_hfield(h)
+    f.set_varying_hfield(1.0, 1.0, 0.7)
+    rel = 1e-12 if precision == qmin.QM_FP64 else 2e-5
+    assert_forces_close(f.compute_forces(g), o.forces(), rel=rel)
+    e_ref, c_ref = o.energy()
+    e_got, c_got = f.compute_energy(g)
+    np.testing.assert_allclose(e_got, e_ref, rtol=1e-12 if precision == qmin.QM_FP64 else 1e-5)
+    np.testing.assert_allclose(c_got[4], c_ref[4], rtol=1e-11 if precision == qmin.QM_FP64 else 1e-5)
+
+
+def test_spatially_varying_field_fire_and_file(tmp_path):
+    shape = (20, 16, 12)
+    o, g, f = build_pair(shape, MULTI, 0, True)
+    h = _random_field(g.n, 9)
+    path = tmp_path / "field_x0y0z0.txt"
+    with open(path, "w") as fh:
+        for i in range(g.n):
+            x, y, z = i % 20, (i // 20) % 16, i // (20 * 16)
+            fh.write("%d %d %d %.17g %.17g %.17g\n" % (x, y, z, h[i, 0], h[i, 1], h[i, 2]))
+    o.set_hfield_varying(h, 1.0, 1.0, 0.5)
+    g.load_hfield(path)
+    f.set_varying_hfield(1.0, 1.0, 0.5)
+    it_ref, tr_ref = o.fire(max_iter=30)
+    m = qmin.FIRE(max_iter=30)
+    m.set_trace(32)
+    m.minimize(f, g)
+    np.testing.assert_allclose(m.trace()[:, 0], tr_ref[:, 0], rtol=1e-10)
+    np.testing.assert_allclose(g.q(), o.q(), rtol=0, atol=1e-11)
+
+
+def test_varying_field_requires_lattice_field():
+    _, g, f = build_pair((12, 10, 9), ONE, 1, False)
+    f.set_varying_hfield(1.0, 1.0, 0.5)
+    with pytest.raises(qmin.QminError):
+        f.compute_forces(g)

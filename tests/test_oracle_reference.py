@@ -235,3 +235,38 @@ def test_distortion_energy_matches_reference_polynomials():
         for i, name in enumerate(("L1", "L2", "L3", "L4", "L6")):
             ref = eval(exprs[name], {name: 1.0}, env)
             np.testing.assert_allclose(e[i], ref, rtol=1e-12, atol=1e-12, err_msg=name)
+
+
+def test_varying_field_oracle_reduces_to_uniform_and_is_local():
+    """The oracle's spatially varying H field: a constant field equals the
+    uniform H field (itself pinned by the reference traces with fields), and a
+    change at one site changes only that site's force."""
+    A, B, C = po.rescaled_phase()
+    S0 = po.uniaxial_s0(A, B, C)
+    shape = (9, 8, 7)
+    n = shape[0] * shape[1] * shape[2]
+    H = (0.3, -0.4, 0.7)
+    outs = []
+    for varying in (False, True):
+        o = po.OracleSystem(*shape)
+        o.random_init(3, S0)
+        o.sphere((4.0, 4.0, 3.0), 2.0, 0, 5.3, S0)
+        o.set_params(A, B, C, 2.32, 0.5, 0.3, semantics=0)
+        if varying:
+            o.set_hfield_varying(np.tile(H, (n, 1)), 1.0, 1.0, 0.5)
+        else:
+            o.set_hfield(H, 1.0, 1.0, 0.5)
+        outs.append((o.forces(), o.energy()[1]))
+    np.testing.assert_allclose(outs[1][0], outs[0][0], rtol=0, atol=1e-15)
+    np.testing.assert_allclose(outs[1][1], outs[0][1], rtol=1e-15)
+    h = np.tile(H, (n, 1))
+    o = po.OracleSystem(*shape)
+    o.random_init(3, S0)
+    o.set_params(A, B, C, 2.32, semantics=1)
+    o.set_hfield_varying(h, 1.0, 1.0, 0.5)
+    f0 = o.forces()
+    h[17] = (1.5, 0.2, -0.3)
+    o.set_hfield_varying(h, 1.0, 1.0, 0.5)
+    f1 = o.forces()
+    changed = np.where(np.abs(f1 - f0).max(axis=1) > 0)[0]
+    assert changed.tolist() == [17]
