@@ -239,6 +239,40 @@ int qm_ensemble_destroy(qm_ensemble* e);
 int qm_ensemble_fire_minimize(qm_ensemble* e, const qm_fire* params, qm_ldg* f, int graphIterations,
                               int* iterations, double* forceMax);
 
+/* ---- learned warm start -----------------------------------------------------
+ * No reference counterpart.  Building blocks of a small 3-D convolutional
+ * network that predicts an initial Q from the geometry; random-start FIRE stays
+ * the fallback.  Activations are channels-last [n][C] BF16 (raw 16-bit words),
+ * C in {16, 32, 64}; weights are [27][cout][cin] (tap = (dz+1)*9 + (dy+1)*3 +
+ * (dx+1), offsets times the dilation, periodic); all pointers are device
+ * memory, work is queued on `stream` (a hipStream_t, NULL = default). */
+/* y[s][co] = bias[co] + sum_{tap,ci} x'[s + dil*off(tap)][ci] wt[tap][co][ci],
+ * x' = max(x, 0) if reluIn; y zeroed where mask[s][co] <= 0 (mask may be NULL);
+ * output to ybf16 (BF16) or yf32 (FP32), exactly one non-NULL */
+int qm_ws_conv_forward(const void* x, int cin, const void* wt, const float* bias, int cout, const void* mask,
+                       void* ybf16, float* yf32, int lx, int ly, int lz, int dil, int reluIn, void* stream);
+/* dw[tap][co][ci] = sum_s g[s][co] x'[s + dil*off(tap)][ci]; part holds
+ * ceil(n/chunk) * 27 * cout * cin floats of per-chunk sums (chunk a multiple of 16) */
+int qm_ws_conv_wgrad(const void* g, int cout, const void* x, int cin, int reluIn, int lx, int ly, int lz, int dil,
+                     float* part, long long chunk, float* dw, void* stream);
+int qm_ws_bias_grad(const void* g, long long n, int cout, float* db, void* stream);
+/* Adam step `step` (1-based) on FP32 weights; refreshes the BF16 copy and, when
+ * wbd16 is not NULL, the mirrored transpose wbd[26-tap][ci][co] used for input gradients */
+int qm_ws_adam(float* w, const float* g, float* m, float* v, long long count, float lr, float b1, float b2, float eps,
+               int step, void* wbf16, void* wbd16, int cout, int cin, void* stream);
+int qm_ws_weights_to_bf16(const float* w, long long count, void* wbf16, void* wbd16, int cout, int cin, void* stream);
+/* g = 2 (y - target) scale on the first five of cy channels at sites with type <= 0,
+ * zero elsewhere; lossPart[blocks] gets per-block sums of squared differences */
+int qm_ws_mse_grad(const float* y, int cy, const float* target, const void* types, long long n, float scale, void* g,
+                   double* lossPart, int blocks, void* stream);
+/* [n][16] BF16 input features of the lattice's geometry and far-field director d */
+int qm_ws_features(qm_lattice* lat, double dx, double dy, double dz, void* out);
+int qm_ws_types(qm_lattice* lat, void* out);
+/* write the first five of cy FP32 channels of y into Q of the lattice's
+ * liquid-crystal sites; velocities cleared */
+int qm_ws_apply(qm_lattice* lat, const float* y, int cy);
+int qm_ws_extract_q(qm_lattice* lat, float* out);
+
 /* ---- kernel timing (HIP events on the library's stream) -------------------
  * When enabled, every launch of the two FIRE kernels is bracketed by events;
  * qm_timing_get reports launches and total milliseconds per kernel:

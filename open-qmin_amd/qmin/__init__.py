@@ -46,6 +46,17 @@ _API = {
     "qm_lattice_set_hfield": [_VP, _D],
     "qm_lattice_load_hfield": [_VP, ctypes.c_char_p],
     "qm_ldg_set_varying_hfield": [_VP, _i, _d, _d, _d],
+    "qm_ws_conv_forward": [_VP, _i, _VP, _VP, _i, _VP, _VP, _VP, _i, _i, _i, _i, _i, _VP],
+    "qm_ws_conv_wgrad": [_VP, _i, _VP, _i, _i, _i, _i, _i, _i, _VP, ctypes.c_longlong, _VP, _VP],
+    "qm_ws_bias_grad": [_VP, ctypes.c_longlong, _i, _VP, _VP],
+    "qm_ws_adam": [_VP, _VP, _VP, _VP, ctypes.c_longlong, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                   ctypes.c_float, _i, _VP, _VP, _i, _i, _VP],
+    "qm_ws_weights_to_bf16": [_VP, ctypes.c_longlong, _VP, _VP, _i, _i, _VP],
+    "qm_ws_mse_grad": [_VP, _i, _VP, _VP, ctypes.c_longlong, ctypes.c_float, _VP, _VP, _i, _VP],
+    "qm_ws_features": [_VP, _d, _d, _d, _VP],
+    "qm_ws_types": [_VP, _VP],
+    "qm_ws_apply": [_VP, _VP, _i],
+    "qm_ws_extract_q": [_VP, _VP],
     "qm_ensemble_create": [ctypes.POINTER(_VP), _i, ctypes.POINTER(_VP)],
     "qm_ensemble_destroy": [_VP],
     "qm_ensemble_fire_minimize": [_VP, _VP, _VP, _i, ctypes.POINTER(ctypes.c_int), _D],
