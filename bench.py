@@ -35,10 +35,13 @@ def phase_constants():
     return A, B, C, S0
 
 
-def build_workload(qmin, L, precision, seed=1234):
+def build_workload(qmin, L, precision, seed=1234, uniform=None):
     A, B, C, S0 = phase_constants()
     lat = qmin.Lattice(L, L, L, precision=precision)
-    lat.random_init(seed, S0)
+    if uniform is None:
+        lat.random_init(seed, S0)
+    else:
+        lat.uniform_init(uniform, S0)
     lat.wall(2, 0, qmin.DEGENERATE_PLANAR, 5.3, S0)
     lat.wall(2, L - 1, qmin.DEGENERATE_PLANAR, 5.3, S0)
     c = L / 2.0
@@ -103,6 +106,33 @@ def time_to_tolerance(qmin, L, mode, fc=1e-8, switch_fc=None, max_iter=400000):
                    polish_iterations=st.iterations - first, polish_seconds=t2 - t1)
     print("time to forceMax<%g (%s): %d iterations, %.2f s" % (fc, mode, st.iterations, t2 - t0), file=sys.stderr,
           flush=True)
+    return out
+
+
+def warm_start(qmin, L, fc=1e-8):
+    """FP64 FIRE to forceMax < fc on the bench workload from the trained
+    network's prediction (committed weights) and from a uniform director along
+    the same far-field direction (x, in the plane of the planar walls); time
+    includes the prediction."""
+    from qmin import warmstart as ws
+    path = os.path.join(REPO, "open-qmin_amd", "warmstart", "weights_64.npz")
+    net = ws.WarmStartNet(seed=0)
+    net.load(path)
+    A, B, C, S0 = phase_constants()
+    out = {}
+    for mode in ("network", "uniform_director"):
+        lat, force = build_workload(qmin, L, qmin.QM_FP64, uniform=(1.0, 0.0, 0.0) if mode != "network" else None)
+        m = qmin.FIRE(fc=fc, max_iter=400000, **fire_params())
+        t0 = time.perf_counter()
+        if mode == "network":
+            net.predict(lat, (1.0, 0.0, 0.0), S0)
+        st = m.minimize(force, lat)
+        t1 = time.perf_counter()
+        e, _ = force.compute_energy(lat)
+        out[mode] = dict(seconds=t1 - t0, iterations=st.iterations, forceMax=st.forceMax,
+                         converged=bool(st.converged), energy_per_site=e / lat.n_active())
+        print("time to forceMax<%g (%s start): %d iterations, %.2f s" % (fc, mode, st.iterations, t1 - t0),
+              file=sys.stderr, flush=True)
     return out
 
 
@@ -234,7 +264,13 @@ def main():
     }
     if not args.no_ttt:
         out["time_to_tolerance"] = compare_modes(qmin, args.size, args.ttt_fc)
-        out["time_to_tolerance"]["warm_start"] = None
+        ws = warm_start(qmin, args.size, args.ttt_fc)
+        t_random = out["time_to_tolerance"]["fp64"]
+        for v in ws.values():
+            v["speedup_vs_random_fp64"] = t_random["seconds"] / v["seconds"]
+            v["energy_rel_diff_vs_random_fp64"] = (v["energy_per_site"] - t_random["energy_per_site"]) / abs(
+                t_random["energy_per_site"])
+        out["time_to_tolerance"]["warm_start"] = ws
     if not args.no_energy_check:
         # forceMax < 1e-8 (= |F|_2 / N) leaves the slow elastic modes unconverged by ~1e-6 in E/N whatever the
         # precision; at 1e-10 the converged energies of the two modes are compared

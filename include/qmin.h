@@ -261,16 +261,17 @@ int qm_ws_bias_grad(const void* g, long long n, int cout, float* db, void* strea
 int qm_ws_adam(float* w, const float* g, float* m, float* v, long long count, float lr, float b1, float b2, float eps,
                int step, void* wbf16, void* wbd16, int cout, int cin, void* stream);
 int qm_ws_weights_to_bf16(const float* w, long long count, void* wbf16, void* wbd16, int cout, int cin, void* stream);
-/* g = 2 (y - target) scale on the first five of cy channels at sites with type <= 0,
- * zero elsewhere; lossPart[blocks] gets per-block sums of squared differences */
-int qm_ws_mse_grad(const float* y, int cy, const float* target, const void* types, long long n, float scale, void* g,
-                   double* lossPart, int blocks, void* stream);
+/* g = 2 (y + offset - target) scale on the first five of cy channels at sites
+ * with type <= 0, zero elsewhere (offset5: host double[5], may be NULL);
+ * lossPart[blocks] gets per-block sums of squared differences */
+int qm_ws_mse_grad(const float* y, int cy, const float* target, const void* types, long long n, float scale,
+                   const double* offset5, void* g, double* lossPart, int blocks, void* stream);
 /* [n][16] BF16 input features of the lattice's geometry and far-field director d */
 int qm_ws_features(qm_lattice* lat, double dx, double dy, double dz, void* out);
 int qm_ws_types(qm_lattice* lat, void* out);
-/* write the first five of cy FP32 channels of y into Q of the lattice's
- * liquid-crystal sites; velocities cleared */
-int qm_ws_apply(qm_lattice* lat, const float* y, int cy);
+/* write the first five of cy FP32 channels of y plus offset5 (host double[5],
+ * may be NULL) into Q of the lattice's liquid-crystal sites; velocities cleared */
+int qm_ws_apply(qm_lattice* lat, const float* y, int cy, const double* offset5);
 int qm_ws_extract_q(qm_lattice* lat, float* out);
 
 /* ---- kernel timing (HIP events on the library's stream) -------------------

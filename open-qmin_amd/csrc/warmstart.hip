@@ -227,11 +227,17 @@ __global__ void weights_to_bf16_kernel(const float* w, long long count, uint16_t
     }
 }
 
+// Offset added to the first five output channels: the network predicts Q as a
+// correction to the uniform far-field state.
+struct Off5 {
+    float v[5];
+};
+
 // Gradient of the mean squared error over liquid-crystal sites (types <= 0) on
-// the first five output channels; per-block partial losses in FP64.
+// the first five output channels (plus the offset); per-block partial losses in FP64.
 __global__ void __launch_bounds__(kWsThreads) mse_grad_kernel(const float* y, int cy, const float* target,
                                                              const int8_t* types, long long n, float scale,
-                                                             uint16_t* g, double* lossPart) {
+                                                             uint16_t* g, double* lossPart, Off5 off) {
     __shared__ double red[kWsThreads];
     double l = 0;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
@@ -239,7 +245,7 @@ __global__ void __launch_bounds__(kWsThreads) mse_grad_kernel(const float* y, in
         for (int c = 0; c < cy; ++c) {
             float gv = 0.f;
             if (lc && c < 5) {
-                const float d = y[i * cy + c] - target[i * 5 + c];
+                const float d = (y[i * cy + c] + off.v[c]) - target[i * 5 + c];
                 l += double(d) * double(d);
                 gv = 2.f * d * scale;
             }
@@ -301,11 +307,11 @@ __global__ void features_kernel(const R* q, const int8_t* types, const DevObject
 
 // Q of liquid-crystal sites from the first five channels of y; velocities zero.
 template <typename R>
-__global__ void apply_kernel(const float* y, int cy, const int8_t* types, long long n, R* q, R* u) {
+__global__ void apply_kernel(const float* y, int cy, const int8_t* types, long long n, R* q, R* u, Off5 off) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
         if (types[i] <= 0) {
 #pragma unroll
-            for (int c = 0; c < 5; ++c) q[c * n + i] = R(y[i * cy + c]);
+            for (int c = 0; c < 5; ++c) q[c * n + i] = R(y[i * cy + c] + off.v[c]);
         }
 #pragma unroll
         for (int c = 0; c < 5; ++c) u[c * n + i] = R(0);
@@ -448,11 +454,18 @@ int qm_ws_weights_to_bf16(const float* w, long long count, void* wbf16, void* wb
     return QM_OK;
 }
 
-int qm_ws_mse_grad(const float* y, int cy, const float* target, const void* types, long long n, float scale, void* g,
-                   double* lossPart, int blocks, void* stream) {
+static Off5 off5(const double* offset5) {
+    Off5 o{};
+    for (int c = 0; c < 5; ++c) o.v[c] = offset5 ? float(offset5[c]) : 0.f;
+    return o;
+}
+
+int qm_ws_mse_grad(const float* y, int cy, const float* target, const void* types, long long n, float scale,
+                   const double* offset5, void* g, double* lossPart, int blocks, void* stream) {
     if (!y || !target || !types || !g || !lossPart || blocks < 1 || cy < 5) return fail(QM_EINVAL, "bad argument");
     hipLaunchKernelGGL(mse_grad_kernel, dim3(blocks), dim3(kWsThreads), 0, static_cast<hipStream_t>(stream), y, cy,
-                       target, static_cast<const int8_t*>(types), n, scale, static_cast<uint16_t*>(g), lossPart);
+                       target, static_cast<const int8_t*>(types), n, scale, static_cast<uint16_t*>(g), lossPart,
+                       off5(offset5));
     HIP_TRY(hipGetLastError());
     return QM_OK;
 }
@@ -482,20 +495,21 @@ int qm_ws_types(qm_lattice* L, void* out) {
     return QM_OK;
 }
 
-// Q of liquid-crystal sites from y (first five of cy channels, FP32, device);
-// velocities are cleared and any pending FIRE velocity mix dropped
-int qm_ws_apply(qm_lattice* L, const float* y, int cy) {
+// Q of liquid-crystal sites from y (first five of cy channels, FP32, device)
+// plus offset5 (host, may be NULL); velocities are cleared and any pending FIRE
+// velocity mix dropped
+int qm_ws_apply(qm_lattice* L, const float* y, int cy, const double* offset5) {
     if (!L || !y || cy < 5) return fail(QM_EINVAL, "bad argument");
     if (int rc = sync_ctrl_to_host(L)) return rc;
     const int blocks = grid_for(L->n);
     if (L->precision == QM_FP64) {
         const StateSet<double> st = state_set<double>(L, L->hCtrl.cur);
         hipLaunchKernelGGL(apply_kernel<double>, dim3(blocks), dim3(kWsThreads), 0, L->stream, y, cy, L->dTypes, L->n,
-                           st.q, st.u);
+                           st.q, st.u, off5(offset5));
     } else {
         const StateSet<float> st = state_set<float>(L, L->hCtrl.cur);
         hipLaunchKernelGGL(apply_kernel<float>, dim3(blocks), dim3(kWsThreads), 0, L->stream, y, cy, L->dTypes, L->n,
-                           st.q, st.u);
+                           st.q, st.u, off5(offset5));
     }
     HIP_TRY(hipGetLastError());
     L->hCtrl.identity = 1;

@@ -52,10 +52,10 @@ _API = {
     "qm_ws_adam": [_VP, _VP, _VP, _VP, ctypes.c_longlong, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                    ctypes.c_float, _i, _VP, _VP, _i, _i, _VP],
     "qm_ws_weights_to_bf16": [_VP, ctypes.c_longlong, _VP, _VP, _i, _i, _VP],
-    "qm_ws_mse_grad": [_VP, _i, _VP, _VP, ctypes.c_longlong, ctypes.c_float, _VP, _VP, _i, _VP],
+    "qm_ws_mse_grad": [_VP, _i, _VP, _VP, ctypes.c_longlong, ctypes.c_float, _D, _VP, _VP, _i, _VP],
     "qm_ws_features": [_VP, _d, _d, _d, _VP],
     "qm_ws_types": [_VP, _VP],
-    "qm_ws_apply": [_VP, _VP, _i],
+    "qm_ws_apply": [_VP, _VP, _i, _D],
     "qm_ws_extract_q": [_VP, _VP],
     "qm_ensemble_create": [ctypes.POINTER(_VP), _i, ctypes.POINTER(_VP)],
     "qm_ensemble_destroy": [_VP],

@@ -7,7 +7,9 @@ include/qmin.h); torch only allocates device buffers and supplies the stream.
 
 Network: four 3x3x3 periodic convolutions with dilations 1, 2, 4, 8 (a
 receptive field of 31 sites) and channels 16 -> 32 -> 64 -> 64 -> 16, ReLU
-between them; the first five output channels are Q.  Inputs (16 channels,
+between them; the first five output channels are the correction to the uniform
+far-field state 3/2 S0 (d d - I/3), so an untrained network predicts a uniform
+start along d.  Inputs (16 channels,
 qm_ws_features): site-type one-hot, the anchoring tensor and planar flag and
 log W of object sites, and the far-field tensor d d - I/3 of the director the
 prediction should adopt far from objects.  Trained on converged lattices
@@ -41,6 +43,17 @@ def _stream():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+def uniform_q(d, S0):
+    """Q of a uniform director d (qTensorFromDirector, 3/2 S0 (d d - I/3))."""
+    d = np.asarray(d, np.float64) / np.linalg.norm(d)
+    return np.array([1.5 * S0 * (d[0] * d[0] - 1 / 3), 1.5 * S0 * d[0] * d[1], 1.5 * S0 * d[0] * d[2],
+                     1.5 * S0 * (d[1] * d[1] - 1 / 3), 1.5 * S0 * d[1] * d[2]])
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
 def far_field_director(q, types):
     """Principal eigenvector of the mean Q over bulk sites (q is [n][5])."""
     bulk = types == 0
@@ -51,10 +64,12 @@ def far_field_director(q, types):
 
 
 class Sample:
-    """Device-resident training pair: features, converged Q, site types."""
+    """Device-resident training pair: features, converged Q, site types, and the
+    uniform far-field Q the network's output corrects."""
 
-    def __init__(self, dims, feats, target, types):
+    def __init__(self, dims, feats, target, types, offset):
         self.dims, self.feats, self.target, self.types = dims, feats, target, types
+        self.offset = np.ascontiguousarray(offset, np.float64)
         self.n = dims[0] * dims[1] * dims[2]
         self.n_lc = int((types.cpu().numpy() <= 0).sum()) if hasattr(types, "cpu") else 0
 
@@ -176,8 +191,8 @@ class WarmStartNet:
         g = torch.empty(sample.n * 16, dtype=torch.int16, device="cuda")
         part = torch.empty(blocks, dtype=torch.float64, device="cuda")
         scale = 1.0 / max(sample.n_lc * 5, 1)
-        _check(self.lib.qm_ws_mse_grad(_ptr(y), 16, _ptr(sample.target), _ptr(sample.types), sample.n, scale, _ptr(g),
-                                       _ptr(part), blocks, _stream()))
+        _check(self.lib.qm_ws_mse_grad(_ptr(y), 16, _ptr(sample.target), _ptr(sample.types), sample.n, scale,
+                                       _dp(sample.offset), _ptr(g), _ptr(part), blocks, _stream()))
         return g, part
 
     def train_step(self, sample, lr=1e-3):
@@ -201,28 +216,37 @@ class WarmStartNet:
                 log("epoch %d: mean squared error %.4e" % (ep + 1, mse))
         return history
 
-    def predict(self, lat, director):
+    def predict(self, lat, director, S0=None):
         """Predicted Q written into the lattice's liquid-crystal sites."""
         torch = _torch()
+        if S0 is None:
+            S0 = workloads.phase_constants()[3]
         feats = lattice_features(lat, director)
         acts = self.forward(feats, lat.shape)
         torch.cuda.synchronize()
-        _check(self.lib.qm_ws_apply(lat.h, _ptr(acts[-1]), 16))
+        _check(self.lib.qm_ws_apply(lat.h, _ptr(acts[-1]), 16, _dp(uniform_q(director, S0))))
         return acts[-1]
 
 
 # ---------------------------------------------------------------- training data
 def training_specs(count, size=64, seed=77):
     """Geometries of the training set: size^3 periodic lattices, half of them
-    with z walls (either anchoring), 0-3 spheres as the ensemble workload."""
+    with z walls (either anchoring), 0-3 spheres as the ensemble workload, and a
+    seeded initial director the minimization starts from uniformly (random on
+    the sphere; in the wall plane for planar walls, along z for homeotropic
+    ones), so that each converged target is a function of the geometry and the
+    far-field director rather than of a random start."""
     rng = np.random.default_rng(seed)
     specs = []
     for _ in range(count):
+        d = rng.normal(size=3)
         spec = {"dims": (size, size, size), "seed": int(rng.integers(1, 2 ** 31 - 1)), "spheres": [], "walls": []}
         if rng.random() < 0.5:
             wt = int(rng.integers(0, 2))
             W = float(math.exp(rng.uniform(math.log(0.5), math.log(20.0))))
             spec["walls"] = [(2, 0, wt, W), (2, size - 1, wt, W)]
+            d = np.array([d[0], d[1], 0.0]) if wt == 1 else np.array([0.0, 0.0, 1.0])
+        spec["director"] = tuple(float(v) for v in d / np.linalg.norm(d))
         for _ in range(int(rng.integers(0, 4))):
             r = float(min(rng.uniform(4.0, 16.0), size / 3.0))
             centre = tuple(float(rng.uniform(0, size)) for _ in range(3))
@@ -234,7 +258,10 @@ def training_specs(count, size=64, seed=77):
 
 def build_lattice(qmin, spec, precision, S0):
     lat = qmin.Lattice(*spec["dims"], precision=precision)
-    lat.random_init(spec["seed"], S0)
+    if "director" in spec:
+        lat.uniform_init(spec["director"], S0)
+    else:
+        lat.random_init(spec["seed"], S0)
     for xyz, plane, wt, W in spec.get("walls", []):
         lat.wall(xyz, plane, wt, W, S0)
     for centre, r, btype, W in spec["spheres"]:
@@ -256,7 +283,7 @@ def make_dataset(qmin, specs, force, fc=1e-7, max_iter=4000, batch=128, log=None
             q = lat.q()
             types = lat.types()
             d = far_field_director(q, types)
-            s = Sample(lat.shape, lattice_features(lat, d), lattice_q(lat), lattice_types(lat))
+            s = Sample(lat.shape, lattice_features(lat, d), lattice_q(lat), lattice_types(lat), uniform_q(d, S0))
             samples.append(s)
         if log:
             log("minimized %d training lattices in %.1f s (iterations mean %.0f, converged %d)"

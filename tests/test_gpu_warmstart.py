@@ -121,13 +121,16 @@ def test_adam_and_loss_kernels():
     gbuf = torch.empty(n * 16, dtype=torch.int16, device="cuda")
     part = torch.empty(8, dtype=torch.float64, device="cuda")
     yt, tt, ty = torch.from_numpy(y).cuda(), torch.from_numpy(t).cuda(), torch.from_numpy(types).cuda()
-    check(lib.qm_ws_mse_grad(ptr(yt), 16, ptr(tt), ptr(ty), n, 0.01, ptr(gbuf), ptr(part), 8, stream()))
+    off = np.array([0.1, -0.2, 0.05, 0.0, 0.3])
+    check(lib.qm_ws_mse_grad(ptr(yt), 16, ptr(tt), ptr(ty), n, 0.01, off.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                             ptr(gbuf), ptr(part), 8, stream()))
     torch.cuda.synchronize()
     lc = types <= 0
-    ref_loss = float(((y[:, :5] - t) ** 2)[lc].sum())
-    assert part.cpu().numpy().sum() == pytest.approx(ref_loss, rel=1e-6)
+    y5 = y[:, :5] + off.astype(np.float32)
+    ref_loss = float(((y5 - t) ** 2)[lc].sum())
+    assert part.cpu().numpy().sum() == pytest.approx(ref_loss, rel=1e-5)
     ref_g = np.zeros((n, 16))
-    ref_g[:, :5] = 2 * (y[:, :5] - t) * 0.01
+    ref_g[:, :5] = 2 * (y5 - t) * 0.01
     ref_g[~lc] = 0
     np.testing.assert_allclose(host_bf16(gbuf).reshape(n, 16), ref_g, rtol=2 ** -8, atol=1e-12)
 
@@ -153,10 +156,12 @@ def test_features_and_apply_round_trip():
     yq = np.tile(np.arange(1, 17, dtype=np.float32) * 0.01, (lat.n, 1))
     y.copy_(torch.from_numpy(yq.reshape(-1)))
     torch.cuda.synchronize()
-    check(qmin.library().qm_ws_apply(lat.h, ptr(y), 16))
+    off = np.array([0.5, 0.0, -0.25, 0.125, 0.0])
+    check(qmin.library().qm_ws_apply(lat.h, ptr(y), 16, off.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
     q1 = lat.q()
     lc = types <= 0
-    np.testing.assert_allclose(q1[lc], np.tile(yq[0, :5], (lc.sum(), 1)), rtol=1e-7)
+    expect = (yq[0, :5] + off.astype(np.float32)).astype(np.float32)
+    np.testing.assert_allclose(q1[lc], np.tile(expect, (lc.sum(), 1)), rtol=1e-7)
     assert np.array_equal(q1[~lc], q0[~lc])
     assert np.abs(lat.v()).max() == 0
 
