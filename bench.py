@@ -232,6 +232,10 @@ def main():
         print("bench.py measures one GPU: this build's hot path runs on a single MI355X "
               "(multi-GPU sharding is a design note only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if not args.no_ttt:
+        # the warm start uses torch for buffers: load its HIP runtime before the
+        # library so that one runtime serves both
+        import torch  # noqa: F401
     import qmin
     qmin.library()
     precision = qmin.QM_FP64 if args.precision == "fp64" else qmin.QM_FP32

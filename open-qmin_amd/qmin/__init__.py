@@ -7,6 +7,7 @@ call runs on the GPU; there is no CPU fallback.  A missing library raises.
 """
 import ctypes
 import os
+import sys
 
 import numpy as np
 
@@ -127,12 +128,14 @@ _API = {
 }
 
 _lib = None
+_torch_loaded_first = False   # torch's HIP runtime was in the process before the library
 
 
 def library(path=LIB_PATH):
     """Load libqmin.so (raises if it has not been built)."""
-    global _lib
+    global _lib, _torch_loaded_first
     if _lib is None:
+        _torch_loaded_first = "torch" in sys.modules
         if not os.path.exists(path):
             raise QminError("libqmin.so not built: run python open-qmin_amd/build.py")
         lib = ctypes.CDLL(path)

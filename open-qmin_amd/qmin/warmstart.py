@@ -34,6 +34,16 @@ def _torch():
     return torch
 
 
+def _check_runtime_order():
+    """torch must be imported before libqmin.so is loaded: torch ships its own
+    HIP runtime, and once the library has loaded the system one first, torch
+    finds no GPU.  Importing torch first makes both use one runtime."""
+    from . import _lib, _torch_loaded_first
+    if _lib is not None and not _torch_loaded_first:
+        raise RuntimeError("import torch before the first qmin call when the warm start is used "
+                           "(one HIP runtime for torch and libqmin.so)")
+
+
 def _ptr(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else None
 
@@ -100,6 +110,7 @@ def lattice_q(lat):
 
 class WarmStartNet:
     def __init__(self, seed=0):
+        _check_runtime_order()
         torch = _torch()
         self.lib = library()
         gen = torch.Generator().manual_seed(seed)

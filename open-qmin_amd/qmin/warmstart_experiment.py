@@ -28,6 +28,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
 
 import numpy as np  # noqa: E402
+import torch  # noqa: E402,F401  (before the library: one HIP runtime)
 
 import qmin  # noqa: E402
 from qmin import warmstart as ws  # noqa: E402
