@@ -1,0 +1,85 @@
+"""The HIP path at the benchmark's full size (256^3), through properties that do
+not depend on size (the oracle checks the same kernels at small sizes):
+
+  * translation equivariance: forces of a periodically shifted state are the
+    shifted forces, bit for bit (the per-site arithmetic does not depend on where
+    a site falls in a tile, a z chunk or the grid);
+  * a uniform nematic at S0 is stationary (no force, FIRE stops at once);
+  * the energy is extensive: a 2x2x2 periodic copy of a 128^3 state has 8 times
+    its energy;
+  * FIRE keeps a 2x2x2 periodic copy exactly periodic, and follows the 128^3
+    run's forceMax to reduction rounding.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as po
+import qmin
+
+pytestmark = pytest.mark.gpu
+
+A, B, C = po.rescaled_phase()
+S0 = po.uniaxial_s0(A, B, C)
+L = 256
+
+
+def grid(q, shape):
+    lx, ly, lz = shape
+    return q.reshape(lz, ly, lx, 5)
+
+
+@pytest.mark.parametrize("precision,params", [(qmin.QM_FP64, (2.32, 0.5, 0.3)), (qmin.QM_FP32, (2.32, 0.0, 0.0))])
+def test_translation_equivariance_256(precision, params):
+    lat = qmin.Lattice(L, L, L, precision=precision)
+    lat.random_init(1234, S0)
+    f = qmin.LandauDeGennesLC(A, B, C, *params, semantics=qmin.SEMANTICS_REF_GPU)
+    F0 = grid(f.compute_forces(lat), (L, L, L))
+    q = grid(lat.q(), (L, L, L))
+    shift = (7, 5, 3)   # (z, y, x) sites
+    lat.set_field(qmin.FIELD_Q, np.roll(q, shift, axis=(0, 1, 2)).reshape(-1, 5))
+    F1 = grid(f.compute_forces(lat), (L, L, L))
+    assert np.array_equal(F1, np.roll(F0, shift, axis=(0, 1, 2)))
+
+
+def test_uniform_nematic_is_stationary_256():
+    lat = qmin.Lattice(L, L, L)
+    lat.uniform_init((0.3, -0.5, 0.8), S0)
+    f = qmin.LandauDeGennesLC(A, B, C, 2.32, 0.5, 0.3, semantics=qmin.SEMANTICS_REF_GPU)
+    F = f.compute_forces(lat)
+    assert np.abs(F).max() < 1e-13
+    st = qmin.FIRE(fc=1e-12, max_iter=100).minimize(f, lat)
+    assert st.iterations == 1 and st.forceMax < 1e-12
+
+
+def test_energy_is_extensive_256():
+    small = qmin.Lattice(128, 128, 128)
+    small.random_init(99, S0)
+    f = qmin.LandauDeGennesLC(A, B, C, 2.32, 0.5, 0.3, semantics=qmin.SEMANTICS_REF_GPU)
+    e_small, c_small = f.compute_energy(small)
+    big = qmin.Lattice(L, L, L)
+    big.set_field(qmin.FIELD_Q, np.tile(grid(small.q(), (128, 128, 128)), (2, 2, 2, 1)).reshape(-1, 5))
+    e_big, c_big = f.compute_energy(big)
+    np.testing.assert_allclose(e_big, 8 * e_small, rtol=1e-12)
+    np.testing.assert_allclose(c_big, 8 * c_small, rtol=1e-11, atol=1e-9 * abs(e_big))
+
+
+def test_fire_keeps_a_periodic_copy_periodic_256():
+    small = qmin.Lattice(128, 128, 128)
+    small.random_init(7, S0)
+    q_small = grid(small.q(), (128, 128, 128))
+    big = qmin.Lattice(L, L, L)
+    big.set_field(qmin.FIELD_Q, np.tile(q_small, (2, 2, 2, 1)).reshape(-1, 5))
+    f = qmin.LandauDeGennesLC(A, B, C, 2.32, 0.5, 0.3, semantics=qmin.SEMANTICS_REF_GPU)
+    ms, mb = qmin.FIRE(max_iter=30), qmin.FIRE(max_iter=30)
+    ms.set_trace(32)
+    mb.set_trace(32)
+    ms.minimize(f, small)
+    mb.minimize(f, big)
+    qb = grid(big.q(), (L, L, L))
+    h = 128
+    assert np.array_equal(qb[:h, :h, :h], qb[h:, :h, :h])
+    assert np.array_equal(qb[:h, :h, :h], qb[:h, h:, h:])
+    assert np.array_equal(qb[:h, :h, :h], qb[h:, h:, h:])
+    # forceMax = |F| / N: the copy has sqrt(8) |F| over 8 N
+    np.testing.assert_allclose(mb.trace()[:, 0] * np.sqrt(8.0), ms.trace()[:, 0], rtol=1e-10)
+    np.testing.assert_allclose(qb[:h, :h, :h], grid(small.q(), (128, 128, 128)), rtol=0, atol=1e-11)
