@@ -123,7 +123,7 @@ __device__ __forceinline__ void central_nbr_derivatives(const R (*q)[5][HP], int
 template <typename R>
 using ConstPtr = const R __attribute__((address_space(4)))*;
 
-template <typename R, int TERMS, int HX, int HP>
+template <typename R, int TERMS, int HX, int HP, bool L6_FROM_LDS>
 __device__ __forceinline__ void fast_raw_force(const ForceConst& fc, ConstPtr<R> W, const R (*q)[5][HP],
                                                int sm, int sc, int sp, int h, const R* qc, R* F) {
 #pragma unroll
@@ -142,12 +142,19 @@ __device__ __forceinline__ void fast_raw_force(const ForceConst& fc, ConstPtr<R>
         ldg_fast_lin_b<HX, HP>(W, q, sm, sc, sp, h, spatial);
     }
     if constexpr (TERMS == kMultiL6) {
-        R x[125];
+        R o6[5] = {R(0), R(0), R(0), R(0), R(0)};
+        if constexpr (L6_FROM_LDS) {
+            // operands read from LDS as each product needs them: fewer live
+            // registers (higher occupancy) for more LDS reads; the better trade
+            // in the FIRE pass, whose memory traffic hides the LDS time
+            ldg_fast_l6<HX, HP>(q, sm, sc, sp, h, qc, o6);
+        } else {
+            R x[125];
 #pragma unroll
-        for (int c = 0; c < 5; ++c) x[c] = qc[c];
-        central_nbr_derivatives<HX, HP>(q, sm, sc, sp, h, x);
-        R o6[5];
-        ldg_bulk_l6(x, o6);
+            for (int c = 0; c < 5; ++c) x[c] = qc[c];
+            central_nbr_derivatives<HX, HP>(q, sm, sc, sp, h, x);
+            ldg_bulk_l6(x, o6);
+        }
 #pragma unroll
         for (int c = 0; c < 5; ++c) spatial[c] += R(fc.L6) * o6[c];
     }
@@ -309,8 +316,8 @@ __device__ __forceinline__ void march_body(const MarchArgs<R>& A, const int logi
                     // inside the loop instead of being hoisted and spilled
                     uint64_t wa = reinterpret_cast<uint64_t>(A.W);
                     asm volatile("" : "+s"(wa));
-                    fast_raw_force<R, TERMS, kHX, kHP>(A.fc, reinterpret_cast<ConstPtr<R>>(wa), sQ, sm, sc, sp, hOwn,
-                                                       q, F);
+                    fast_raw_force<R, TERMS, kHX, kHP, MODE == kFire>(A.fc, reinterpret_cast<ConstPtr<R>>(wa), sQ, sm,
+                                                                      sc, sp, hOwn, q, F);
                     if (A.hvar) {
                         R fv[5];
                         varying_field_force(A.hvar, n, i, A.hvarP, fv);

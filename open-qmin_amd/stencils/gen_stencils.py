@@ -227,6 +227,56 @@ def gen_fast(tables):
     return dev, host
 
 
+L6_FENCE_EVERY = int(os.environ.get("QMIN_L6_FENCE_EVERY", "1"))
+
+
+def l6_operand(k):
+    """Expression of stencil input k read straight from the LDS ring (q[slot][c][pos])
+    or the site's own registers (qc): input layout as the tables, x[0..4] the site,
+    x[5 + 5 d + c] neighbour d's Q, x[35 + 15 d + 5 axis + c] its central derivative
+    along a transverse axis; neighbours in the order xd, xu, yd, yu, zd, zu."""
+    pos = [("sc", "h - 1"), ("sc", "h + 1"), ("sc", "h - HX"), ("sc", "h + HX"), ("sm", "h"), ("sp", "h")]
+    if k < 5:
+        return "qc[%d]" % k
+    if k < 35:
+        d, c = divmod(k - 5, 5)
+        slot, at = pos[d]
+        return "q[%s][%d][%s]" % (slot, c, at)
+    d, rest = divmod(k - 35, 15)
+    ax, c = divmod(rest, 5)
+    assert ax != d // 2, "along-axis neighbour derivatives are not inputs"
+    slot, at = pos[d]
+    if ax == 0:
+        up, dn = (slot, "%s + 1" % at), (slot, "%s - 1" % at)
+    elif ax == 1:
+        up, dn = (slot, "%s + HX" % at), (slot, "%s - HX" % at)
+    else:
+        up, dn = ("sp", at), ("sm", at)
+    return "R(0.5) * (q[%s][%d][%s] - q[%s][%d][%s])" % (up[0], c, up[1], dn[0], c, dn[1])
+
+
+def gen_fast_l6(entries):
+    """L6 bulk operator reading its operands from LDS as it goes: terms grouped by
+    the first factor (a Q value), with scheduling fences between groups so that
+    only the five accumulators and a few operands are live at a time."""
+    groups = {}
+    for comp, coef, a, b in entries:
+        groups.setdefault(a, []).append((comp, coef, b))
+    lines = ["template <int HX, int HP, typename R>",
+             "__device__ __forceinline__ void ldg_fast_l6(const R (*q)[5][HP], int sm, int sc, int sp, int h,"
+             " const R* qc, R* acc) {"]
+    for g, a in enumerate(sorted(groups)):
+        if g and g % L6_FENCE_EVERY == 0:
+            lines.append("    __builtin_amdgcn_sched_barrier(0);")
+        lines.append("    {")
+        lines.append("        const R xa = %s;" % l6_operand(a))
+        for comp, coef, b in groups[a]:
+            lines.append("        acc[%d] += %s * (xa * (%s));" % (comp, coef_literal(coef), l6_operand(b)))
+        lines.append("    }")
+    lines.append("}")
+    return lines
+
+
 def main():
     with open(TABLES) as fh:
         doc = json.load(fh)
@@ -240,6 +290,7 @@ def main():
     body += gen_boundary(tables, cases)
     fast_dev, fast_host = gen_fast(tables)
     body += [""] + fast_dev
+    body += [""] + gen_fast_l6(tables["L6"]["bulk"])
     host_text = "\n".join(fast_host) + "\n"
     host_out = os.path.join(os.path.dirname(OUT), "ldg_fast_host.inc")
     if not os.path.exists(host_out) or open(host_out).read() != host_text:
