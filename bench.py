@@ -178,10 +178,87 @@ def ensemble_throughput(qmin, count=1024, fc=1e-7, max_iter=10000, sequential_sa
                            "same_iterations": [int(i) for i in its[:sequential_sample]] == seq_its}}
 
 
+# Minimizers of the full-elasticity workload: (name, class, parameters, iteration
+# cap of each phase).  Step sizes: Nesterov's 1e-2 is the largest of {1e-4, 1e-3,
+# 1e-2, 2e-2, 5e-2} that stays finite on this workload at 128^3; FIRE's dt_max is
+# the ensemble's.  Online L-BFGS (no line search, energyMinimizerLoLBFGS.cpp:104-187)
+# converges at 128^3 with dt = 5e-4 but diverges at 384^3 for dt >= 1e-4 (after
+# about 780 iterations at 1e-4, 100 at 2e-4); at 5e-5 it stays finite and runs a
+# fixed number of iterations.  Adam keeps steps of size ~dt however small the
+# force (m/sqrt(s) does not shrink with F): at a constant dt it stalls near
+# forceMax ~ 1e-6 and also runs a fixed number of iterations.
+FULL_ELASTICITY_MINIMIZERS = (
+    ("FIRE", "FIRE", dict(dt=5e-4, dt_max=2e-2), 20000),
+    ("Nesterov", "Nesterov", dict(dt=1e-2, mu=0.01), 20000),
+    ("LoLBFGS(m=8)", "LoLBFGS", dict(m=8, dt=5e-5), 500),
+    ("Adam", "Adam", dict(dt=1e-3), 1000),
+)
+
+
+def full_elasticity(qmin, L=384, fc=1e-8):
+    """Each minimizer from the same seeded start of the full-elasticity workload
+    (qmin.workloads.full_elasticity: L1..L4, L6, E field, four spheres) until
+    forceMax < fc or its iteration cap, in FP64 and mixed: FP32 state until
+    forceMax < fc, then the state is converted on the device and FP64 polishes
+    to fc (FIRE continues with its dt and alpha; the other minimizers restart
+    their history in FP64, as a new minimizer on the converted lattice)."""
+    from qmin import workloads
+    out = {"size": L, "forceMax_below": fc}
+    for name, cls, params, cap in FULL_ELASTICITY_MINIMIZERS:
+        res = {"parameters": dict(params), "iteration_cap": cap}
+        for mode in ("fp64", "mixed"):
+            res[mode] = _full_elasticity_run(qmin, workloads, L, fc, cls, params, cap, mode)
+            r = res[mode]
+            print("full elasticity %d^3 %s %s: %s" % (L, name, mode, r.get("error") or "%d iterations, %.2f s, forceMax "
+                                                      "%.3g" % (r["iterations"], r["seconds"], r["forceMax"])),
+                  file=sys.stderr, flush=True)
+        if "error" not in res["fp64"] and "error" not in res["mixed"]:
+            res["mixed_speedup"] = res["fp64"]["seconds"] / res["mixed"]["seconds"]
+            res["energy_rel_diff_mixed_vs_fp64"] = abs(res["mixed"]["energy_per_site"] -
+                                                       res["fp64"]["energy_per_site"]) / abs(
+                res["fp64"]["energy_per_site"])
+        out[name] = res
+    return out
+
+
+def _full_elasticity_run(qmin, workloads, L, fc, cls, params, cap, mode):
+    """One minimizer in one mode on a fresh full-elasticity lattice."""
+    lat, force = workloads.full_elasticity(qmin, L, qmin.QM_FP64 if mode == "fp64" else qmin.QM_FP32)
+    m = getattr(qmin, cls)(fc=fc, max_iter=cap, **params)
+    t0 = time.perf_counter()
+    try:
+        st = m.minimize(force, lat)
+    except qmin.QminError as e:
+        st = m.state()
+        return {"error": str(e), "iterations": st.iterations if cls == "FIRE" else st[0],
+                "seconds": time.perf_counter() - t0}
+    t1 = time.perf_counter()
+    its, fmax = (st.iterations, st.forceMax) if cls == "FIRE" else st
+    r = {"fp32_iterations": its, "fp32_seconds": t1 - t0} if mode == "mixed" else {}
+    if mode == "mixed" and fmax < fc:
+        lat.set_precision(qmin.QM_FP64)
+        if cls == "FIRE":
+            m.set_max_iterations(its + cap)
+            st = m.minimize(force, lat)
+            its, fmax = st.iterations, st.forceMax
+        else:
+            m2 = getattr(qmin, cls)(fc=fc, max_iter=cap, **params)
+            p_its, fmax = m2.minimize(force, lat)
+            its += p_its
+    t2 = time.perf_counter()
+    e, _ = force.compute_energy(lat)
+    r.update(iterations=its, seconds=t2 - t0, forceMax=fmax, converged=bool(fmax < fc),
+             energy_per_site=e / lat.n_active(), ms_per_iteration=1e3 * (t2 - t0) / max(its, 1))
+    if mode == "mixed":
+        # no polish when the FP32 phase stopped at its cap
+        r.update(polish_iterations=its - r["fp32_iterations"], polish_seconds=t2 - t1)
+    return r
+
+
 def measured_traffic(size, precision):
     """HBM bytes per fused-pass dispatch from the committed rocprofv3 counter
     summary of this workload, when it covers this size and precision."""
-    path = os.path.join(REPO, "profiles", "march_kernel_fire_%s_%d.traffic.json" % (precision, size))
+    path = os.path.join(REPO, "profiles", "march_wide_kernel_fire_%s_%d.traffic.json" % (precision, size))
     if not os.path.exists(path):
         return None
     with open(path) as fh:
@@ -225,6 +302,9 @@ def main():
     ap.add_argument("--ttt-fc", type=float, default=1e-8, help="forceMax tolerance of the time-to-tolerance runs")
     ap.add_argument("--ensemble", action="store_true",
                     help="also measure the 1024-lattice ensemble workload (about 2 minutes)")
+    ap.add_argument("--full-elasticity", action="store_true",
+                    help="also run every minimizer to tolerance on the 384^3 full-elasticity workload in FP64 and "
+                         "mixed precision (several minutes)")
     ap.add_argument("--no-energy-check", action="store_true",
                     help="skip the FP64-vs-mixed converged-energy comparison at forceMax < 1e-10")
     args = ap.parse_args()
@@ -261,8 +341,9 @@ def main():
                    "sites": r["n"], "boundary_sites": r["n_boundary"], "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": measured_traffic(args.size, args.precision),
-                     "kernel": "march_kernel (fused FIRE lattice pass)", "kernel_ms": r["march_ms"],
-                     "boundary_kernel_ms": r["boundary_ms"], "bytes_per_site": bps},
+                     "kernel": "fused FIRE lattice pass (%s)" % (
+                         "march_wide_kernel, 32x16 tile" if args.size >= 128 else "march_kernel, 32x8 / 64x4 tile"),
+                     "kernel_ms": r["march_ms"], "boundary_kernel_ms": r["boundary_ms"], "bytes_per_site": bps},
         "iteration_hbm_frac": value * bps / (PEAK_HBM_GBS * 1e9),
         "forceMax_after": r["forceMax"],
     }
@@ -281,6 +362,8 @@ def main():
         out["converged_energy_check"] = compare_modes(qmin, args.size, 1e-10, switch_fc=1e-8)
     if args.ensemble:
         out["ensemble"] = ensemble_throughput(qmin)
+    if args.full_elasticity:
+        out["full_elasticity"] = full_elasticity(qmin, fc=args.ttt_fc)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

@@ -193,7 +193,7 @@ __device__ __forceinline__ void march_body(const MarchArgs<R>& A, const int logi
     using T = Tile<kTileTX[TILE], kTileTY[TILE]>;
     constexpr int kTX = T::TX, kTY = T::TY, kHX = T::HX, kHY = T::HY, kHP = T::HP, kHalo = T::Halo;
     __shared__ R sQ[3][5][kHP];
-    __shared__ double sRed[kThreads / 64][3];
+    __shared__ double sRed[T::Threads / 64][3];
 
     Ctrl* ctrl = A.ctrl;
     if ((MODE == kFire || A.gate) && ctrl->done) return;
@@ -361,7 +361,7 @@ __device__ __forceinline__ void march_body(const MarchArgs<R>& A, const int logi
         cCur = cNext;
     }
     if (MODE == kFire) {
-        block_sum3<kThreads>(pF, pP, pV, sRed);
+        block_sum3<T::Threads>(pF, pP, pV, sRed);
         if (tid == 0) {
             A.partials[3 * (long long)slot + 0] = pF;
             A.partials[3 * (long long)slot + 1] = pP;
@@ -379,19 +379,34 @@ __device__ __forceinline__ int xcd_logical(int b, int nb) {
 }
 
 template <typename R, int MODE, int TERMS, int TILE>
-__global__ void __launch_bounds__(kThreads) march_kernel(MarchArgs<R> A) {
+__global__ void __launch_bounds__(kTileThreads[TILE]) march_kernel(MarchArgs<R> A) {
     march_body<R, MODE, TERMS, TILE>(A, xcd_logical(blockIdx.x, A.nblocks), blockIdx.x);
+}
+
+// The 32 x 16 tile (512 threads, 8 waves): two work-groups per CU need at least
+// four waves per SIMD, i.e. at most 128 VGPRs.
+template <typename R, int MODE, int TERMS>
+__global__ void __launch_bounds__(kTileThreads[kTile32x16], 4) march_wide_kernel(MarchArgs<R> A) {
+    march_body<R, MODE, TERMS, kTile32x16>(A, xcd_logical(blockIdx.x, A.nblocks), blockIdx.x);
 }
 
 // The fused pass over many lattices in one launch: map[b] = {lattice, logical
 // tile} for hardware work-group b, laid out by the host in XCD-aware order.
 template <typename R, int MODE, int TERMS, int TILE>
-__global__ void __launch_bounds__(kThreads) march_batch_kernel(const MarchArgs<R>* __restrict__ descs,
+__global__ void __launch_bounds__(kTileThreads[TILE]) march_batch_kernel(const MarchArgs<R>* __restrict__ descs,
                                                                const int2* __restrict__ map) {
     // uniform over the work-group: read once, keep in scalar registers
     const int lat = __builtin_amdgcn_readfirstlane(map[blockIdx.x].x);
     const int logical = __builtin_amdgcn_readfirstlane(map[blockIdx.x].y);
     march_body<R, MODE, TERMS, TILE>(descs[lat], logical, logical);
+}
+
+template <typename R, int MODE, int TERMS>
+__global__ void __launch_bounds__(kTileThreads[kTile32x16], 4) march_wide_batch_kernel(
+    const MarchArgs<R>* __restrict__ descs, const int2* __restrict__ map) {
+    const int lat = __builtin_amdgcn_readfirstlane(map[blockIdx.x].x);
+    const int logical = __builtin_amdgcn_readfirstlane(map[blockIdx.x].y);
+    march_body<R, MODE, TERMS, kTile32x16>(descs[lat], logical, logical);
 }
 
 // ---------------------------------------------------------------- boundary pass
@@ -811,8 +826,11 @@ __global__ void defect_kernel(const R* q, const int8_t* types, long long n, int 
 
 template <typename R, int MODE, int TERMS>
 static const void* march_fn_tile(int tile) {
-    return tile == kTile64x4 ? reinterpret_cast<const void*>(&march_kernel<R, MODE, TERMS, kTile64x4>)
-                             : reinterpret_cast<const void*>(&march_kernel<R, MODE, TERMS, kTile32x8>);
+    switch (tile) {
+        case kTile64x4: return reinterpret_cast<const void*>(&march_kernel<R, MODE, TERMS, kTile64x4>);
+        case kTile32x16: return reinterpret_cast<const void*>(&march_wide_kernel<R, MODE, TERMS>);
+        default: return reinterpret_cast<const void*>(&march_kernel<R, MODE, TERMS, kTile32x8>);
+    }
 }
 
 template <typename R, int MODE>
@@ -837,8 +855,11 @@ static void launch_boundary_t(const BoundaryArgs<R>& a, int blocks, hipStream_t 
 
 template <typename R, int MODE, int TERMS>
 static const void* march_batch_fn_tile(int tile) {
-    return tile == kTile64x4 ? reinterpret_cast<const void*>(&march_batch_kernel<R, MODE, TERMS, kTile64x4>)
-                             : reinterpret_cast<const void*>(&march_batch_kernel<R, MODE, TERMS, kTile32x8>);
+    switch (tile) {
+        case kTile64x4: return reinterpret_cast<const void*>(&march_batch_kernel<R, MODE, TERMS, kTile64x4>);
+        case kTile32x16: return reinterpret_cast<const void*>(&march_wide_batch_kernel<R, MODE, TERMS>);
+        default: return reinterpret_cast<const void*>(&march_batch_kernel<R, MODE, TERMS, kTile32x8>);
+    }
 }
 
 template <typename R, int MODE>
@@ -857,7 +878,7 @@ hipError_t launch_march_batch(const MarchArgs<R>* descs, const int2* map, int bl
     const void* fn = mode == kFire ? march_batch_fn_terms<R, kFire>(terms, tile)
                                    : march_batch_fn_terms<R, kForce>(terms, tile);
     void* args[] = {&descs, &map};
-    return hipLaunchKernel(fn, dim3(blocks), dim3(kThreads), args, 0, s);
+    return hipLaunchKernel(fn, dim3(blocks), dim3(kTileThreads[tile]), args, 0, s);
 }
 
 template <typename R>
@@ -884,7 +905,7 @@ hipError_t launch_fire_finalize_batch(const FinalizeDesc* descs, EnsembleStatus*
 template <typename R>
 hipError_t launch_march(const MarchArgs<R>& a, int mode, int terms, int tile, hipStream_t s) {
     void* args[] = {const_cast<MarchArgs<R>*>(&a)};
-    return hipLaunchKernel(march_fn<R>(mode, terms, tile), dim3(a.nblocks), dim3(kThreads), args, 0, s);
+    return hipLaunchKernel(march_fn<R>(mode, terms, tile), dim3(a.nblocks), dim3(kTileThreads[tile]), args, 0, s);
 }
 
 template <typename R>
@@ -908,7 +929,7 @@ void launch_boundary(const BoundaryArgs<R>& a, int mode, int terms, hipStream_t 
 template <typename R>
 int march_blocks_per_cu(int mode, int terms, int tile) {
     int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, march_fn<R>(mode, terms, tile), kThreads, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, march_fn<R>(mode, terms, tile), kTileThreads[tile], 0) !=
         hipSuccess)
         return 1;
     return blocks < 1 ? 1 : blocks;

@@ -417,7 +417,7 @@ using namespace qmin;
 // Device state common to the three minimizers: vectors of 5 n reals bound to
 // one lattice, scalars, partial sums and the forceMax trace.
 struct VecMinimizer {
-    qm_lattice* bound = nullptr;
+    uint64_t bound = 0;   // serial of the lattice the vectors belong to (0: none)
     long long boundN = 0;
     int boundPrecision = -1;
     std::vector<void*> vecs;
@@ -444,12 +444,12 @@ struct VecMinimizer {
         if (dPartials) (void)hipFree(dPartials);
         dScal = nullptr;
         dPartials = nullptr;
-        bound = nullptr;
+        bound = 0;
         boundN = 0;
         boundPrecision = -1;
     }
     bool needs_bind(const qm_lattice* L) const {
-        return bound != L || boundN != L->n || boundPrecision != L->precision;
+        return bound != L->serial || boundN != L->n || boundPrecision != L->precision;
     }
     // (re)allocates nv zeroed vectors and nscal zeroed scalars for lattice L
     int bind(qm_lattice* L, int nv, int nscal) {
@@ -468,7 +468,7 @@ struct VecMinimizer {
         HIP_TRY(hipMemsetAsync(dScal, 0, std::max(nscal, 1) * sizeof(double), L->stream));
         scalCount = nscal;
         HIP_TRY(hipMalloc(&dPartials, (size_t)kVecMaxBlocks * kMaxDots * sizeof(double)));
-        bound = L;
+        bound = L->serial;
         boundN = L->n;
         boundPrecision = L->precision;
         return QM_OK;
@@ -635,6 +635,8 @@ int qm_adam_minimize(qm_adam* m, qm_ldg* f, qm_lattice* L) {
         // initializeFromModel (energyMinimizerAdam.cpp:9-23): zero moments, iteration 0
         if (int rc = m->bind(L, 2, 2)) return rc;
         m->iterations = 0;
+        m->b1t = m->b1;
+        m->b2t = m->b2;
     }
     m->forceMax = 110.0;
     if (int rc = begin_run(*m, L, f, m->iterations >= m->maxIter)) return rc;
@@ -664,6 +666,7 @@ int qm_adam_get_state(const qm_adam* m, int* iterations, double* forceMax) {
 struct qm_nesterov : VecMinimizer {
     // setNesterovAGParameters defaults (energyMinimizerNesterovAG.h:22-27)
     double dt = 1e-4, mu = 0.01;
+    double mu0 = 0.01;   // mu as set: the scheduled momentum restarts from it
     double lambda = 1.0;
     int scheduled = 0;
 };
@@ -706,6 +709,7 @@ int qm_nesterov_set_parameters(qm_nesterov* m, double dt, double mu, double forc
     if (!m) return fail(QM_EINVAL, "null minimizer");
     m->dt = dt;
     m->mu = mu;
+    m->mu0 = mu;
     m->fc = forceCutoff;
     m->scheduled = scheduledMomentum != 0;
     return QM_OK;
@@ -720,11 +724,18 @@ int qm_nesterov_set_max_iterations(qm_nesterov* m, int maxIterations) {
 int qm_nesterov_minimize(qm_nesterov* m, qm_ldg* f, qm_lattice* L) {
     if (!m || !f || !L) return fail(QM_EINVAL, "null handle");
     m->forceMax = 110.0;
+    // a new lattice (or precision) starts a new run: iteration 0, momentum as set
+    const bool rebind = m->needs_bind(L);
+    if (rebind) {
+        if (int rc = m->bind(L, 1, 2)) return rc;
+        m->iterations = 0;
+        m->lambda = 1.0;
+        m->mu = m->mu0;
+    }
     if (int rc = begin_run(*m, L, f, m->iterations >= m->maxIter)) return rc;
-    if (m->needs_bind(L)) {
+    if (rebind) {
         // initializeFromModel (energyMinimizerNesterovAG.cpp:31-37): the
         // alternate sequence starts as a copy of the positions
-        if (int rc = m->bind(L, 1, 2)) return rc;
         HIP_TRY(hipMemcpyAsync(m->vecs[0], L->dSet[L->hCtrl.cur], 5 * (size_t)L->n * L->realSize,
                                hipMemcpyDeviceToDevice, L->stream));
     }
@@ -905,6 +916,8 @@ int qm_lolbfgs_minimize(qm_lolbfgs* M, qm_ldg* f, qm_lattice* L) {
         // initializeFromModel (energyMinimizerLoLBFGS.cpp:32-52): zero history
         if (int rc = M->bind(L, 1 + 2 * M->m, LbLayout{M->m}.size())) return rc;
         M->fresh = true;
+        M->iterations = 0;
+        M->cur = 0;
     }
     // at least one iteration per call (energyMinimizerLoLBFGS.cpp:199-201)
     if (int rc = begin_run(*M, L, f, false)) return rc;

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -103,6 +104,7 @@ static int refresh_types(qm_lattice* L) {
     const int lx = L->lx, ly = L->ly, lz = L->lz;
     std::vector<int8_t> t8(n), cls(n);
     std::vector<int32_t> sites;
+    int64_t active = 0;
     auto T = [&](int x, int y, int z) { return L->types[site_of(L, wrapc(x, lx), wrapc(y, ly), wrapc(z, lz))]; };
     for (int z = 0; z < lz; ++z)
         for (int y = 0; y < ly; ++y)
@@ -126,6 +128,7 @@ static int refresh_types(qm_lattice* L) {
                 }
                 cls[i] = (int8_t)c;
                 if (c < 0) sites.push_back((int32_t)i);
+                active += t <= 0;
             }
     HIP_TRY(hipMemcpy(L->dTypes, t8.data(), n, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(L->dClass, cls.data(), n, hipMemcpyHostToDevice));
@@ -139,6 +142,7 @@ static int refresh_types(qm_lattice* L) {
         HIP_TRY(hipMalloc(&L->dSites, sites.size() * sizeof(int32_t)));
         HIP_TRY(hipMemcpy(L->dSites, sites.data(), sites.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     }
+    L->nActive = active;
     L->typesDirty = false;
     return QM_OK;
 }
@@ -196,7 +200,9 @@ int qm_lattice_create(int lx, int ly, int lz, int precision, qm_lattice** out) {
     if (precision != QM_FP64 && precision != QM_FP32) return fail(QM_EINVAL, "precision must be QM_FP64 or QM_FP32");
     const long long n = (long long)lx * ly * lz;
     if (n > (1LL << 31) - 1) return fail(QM_EINVAL, "lattice exceeds 2^31-1 sites");
+    static std::atomic<uint64_t> serials{0};
     std::unique_ptr<qm_lattice> L(new qm_lattice);
+    L->serial = ++serials;
     L->lx = lx;
     L->ly = ly;
     L->lz = lz;
@@ -319,7 +325,7 @@ int qm_lattice_load_hfield(qm_lattice* L, const char* path) {
 }
 
 int qm_lattice_set_tile(qm_lattice* L, int tile) {
-    if (!L || tile < -1 || tile >= kNumTiles) return fail(QM_EINVAL, "tile is -1 (automatic), 0 (32x8) or 1 (64x4)");
+    if (!L || tile < -1 || tile >= kNumTiles) return fail(QM_EINVAL, "tile is -1 (automatic), 0 (32x8), 1 (64x4) or 2 (32x16)");
     L->tileOverride = tile;
     return QM_OK;
 }
@@ -514,9 +520,7 @@ int qm_lattice_get_neighbors(const qm_lattice* L, int32_t* nb) {
 
 int qm_lattice_n_active(const qm_lattice* L, int64_t* n) {
     if (!L || !n) return fail(QM_EINVAL, "bad argument");
-    int64_t c = 0;
-    for (long long i = 0; i < L->n; ++i) c += L->types[i] <= 0;
-    *n = c;
+    *n = qmin::n_active(L);
     return QM_OK;
 }
 
@@ -721,9 +725,12 @@ static int upload_weights(qm_lattice* L, const qm_ldg* f) {
 }
 
 // Tile shape of the fused pass: the one set on the lattice, else the measured
-// better default per precision (32 x 8 for FP64, 64 x 4 for FP32).
+// better default: 32 x 16 on lattices at least 128 sites wide in x and y (256^3:
+// 2% faster than 32 x 8 in FP64, 6% faster than 64 x 4 in FP32); on smaller ones
+// 32 x 8 for FP64 and 64 x 4 for FP32, which waste fewer threads on partial tiles.
 int qmin::active_tile(const qm_lattice* L) {
     if (L->tileOverride >= 0) return L->tileOverride;
+    if (L->lx >= 128 && L->ly >= 128) return kTile32x16;
     return L->precision == QM_FP64 ? kTile32x8 : kTile64x4;
 }
 
@@ -936,11 +943,8 @@ int qmin::force_pass(qm_lattice* L, const qm_ldg* f, bool gated) {
                                    : run_pass<float>(L, f, kForce, fire, gated);
 }
 
-int64_t qmin::n_active(const qm_lattice* L) {
-    int64_t c = 0;
-    for (long long i = 0; i < L->n; ++i) c += L->types[i] <= 0;
-    return c;
-}
+// every change of the types ends in refresh_types, which counts them
+int64_t qmin::n_active(const qm_lattice* L) { return L->nActive; }
 
 int qm_ldg_compute_forces(qm_ldg* f, qm_lattice* L) {
     if (int rc = prepare(L, f)) return rc;

@@ -5,23 +5,26 @@
 
 namespace qmin {
 
-// Work-group of the fused pass: 256 threads (4 wave64), one site each, in a
-// TX x TY tile with a one-site halo in x and y; the work-group marches along z
-// through a chunk.  Two tile shapes are compiled: 32 x 8 and 64 x 4 (wider rows
-// use whole cache lines for the side halo; more halo sites per tile).
-constexpr int kThreads = 256;
-enum TileId { kTile32x8 = 0, kTile64x4 = 1, kNumTiles = 2 };
-constexpr int kTileTX[kNumTiles] = {32, 64};
-constexpr int kTileTY[kNumTiles] = {8, 4};
+// Work-group of the fused pass: one thread per site of a TX x TY tile with a
+// one-site halo in x and y; the work-group marches along z through a chunk.
+// Three tile shapes are compiled: 32 x 8 and 64 x 4 (256 threads; wider rows use
+// whole cache lines for the side halo but have more halo sites per tile) and
+// 32 x 16 (512 threads, 73 KB of LDS in FP64: a fifth fewer halo sites per site).
+constexpr int kThreads = 256;   // work-group size of the streaming kernels
+enum TileId { kTile32x8 = 0, kTile64x4 = 1, kTile32x16 = 2, kNumTiles = 3 };
+constexpr int kTileTX[kNumTiles] = {32, 64, 32};
+constexpr int kTileTY[kNumTiles] = {8, 4, 16};
+constexpr int kTileThreads[kNumTiles] = {32 * 8, 64 * 4, 32 * 16};
 
 template <int TX_, int TY_>
 struct Tile {
     static constexpr int TX = TX_, TY = TY_;
+    static constexpr int Threads = TX * TY;     // one site per thread
     static constexpr int HX = TX + 2, HY = TY + 2;
     static constexpr int HP = HX * HY;          // sites of one halo-padded plane
     static constexpr int Halo = HP - TX * TY;   // perimeter sites of one plane
-    static_assert(TX * TY == kThreads, "one site per thread");
-    static_assert(Halo <= kThreads, "one halo site per thread at most");
+    static_assert(Threads % 64 == 0, "whole wave64s");
+    static_assert(Halo <= Threads, "one halo site per thread at most");
 };
 
 constexpr int kMaxObjects = 127;        // types are int8 on the device

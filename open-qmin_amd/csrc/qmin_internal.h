@@ -21,6 +21,7 @@ struct HostObject {
 };
 
 struct qm_lattice {
+    uint64_t serial = 0;   // unique per lattice of the process (addresses can be reused)
     int lx = 0, ly = 0, lz = 0, precision = QM_FP64;
     long long n = 0;
     size_t realSize = 8;
@@ -41,6 +42,7 @@ struct qm_lattice {
     double* dTrace = nullptr;
     int traceCap = 0;
     bool typesDirty = true;
+    int64_t nActive = 0;      // sites with type <= 0, counted when the types change
     int zchunk = 16;          // z extent one work-group marches through
     int zchunkOverride = 0;   // fixed z chunk (0 = choose from occupancy)
     int tileOverride = -1;    // tile shape of the fused pass (-1 = by precision)

@@ -5,6 +5,10 @@
   python open-qmin_amd/qmin/profile_run.py copy [--mib 512] [--steps 10]
       a plain device-to-device copy of known size, used to calibrate the
       FETCH_SIZE / WRITE_SIZE counters (bytes the counter reports per byte moved).
+  python open-qmin_amd/qmin/profile_run.py minimizers --size 384 [--precision fp64]
+      the full-elasticity workload (qmin.workloads.full_elasticity: L1..L4 and
+      L6, a uniform E field, four homeotropic spheres); times per iteration of
+      FIRE, Adam, Nesterov and online L-BFGS (memory 8).
 """
 import argparse
 import os
@@ -16,9 +20,34 @@ sys.path.insert(0, os.path.dirname(HERE))
 sys.path.insert(0, REPO)
 
 
+def minimizers(qmin, L, precision, name):
+    import time
+    from qmin import workloads
+    lat, f = workloads.full_elasticity(qmin, L, precision)
+    print("%d^3 %s, %d boundary sites" % (L, name, lat.n_boundary()), flush=True)
+    runs = (("FIRE", qmin.FIRE(dt=5e-4, dt_max=2e-2, fc=1e-12, max_iter=10 ** 9), 20),
+            ("Adam", qmin.Adam(dt=1e-4, fc=1e-12, max_iter=10 ** 9), 10),
+            ("Nesterov", qmin.Nesterov(dt=1e-4, mu=0.01, fc=1e-12, max_iter=10 ** 9), 10),
+            ("LoLBFGS(m=8)", qmin.LoLBFGS(m=8, dt=1e-4, fc=1e-12, max_iter=10 ** 9), 5))
+    for label, m, its in runs:
+        if label == "FIRE":
+            m.run_iterations(f, lat, 3)
+            t0 = time.perf_counter()
+            m.run_iterations(f, lat, its)
+        else:
+            m.set_max_iterations(2)
+            m.minimize(f, lat)
+            m.set_max_iterations(2 + its)
+            t0 = time.perf_counter()
+            m.minimize(f, lat)
+        t1 = time.perf_counter()
+        print("%s: %.3f ms per iteration, %.3e site-updates/s" % (label, (t1 - t0) / its * 1e3, lat.n * its / (t1 - t0)),
+              flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["fire", "copy"])
+    ap.add_argument("what", choices=["fire", "copy", "minimizers"])
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--steps", type=int, default=10)
@@ -42,6 +71,9 @@ def main():
               % (args.mib, args.steps, ms, 2 * args.mib * (1 << 20) / (ms * 1e-3) / 1e9))
         return
     import qmin
+    if args.what == "minimizers":
+        minimizers(qmin, args.size, qmin.QM_FP64 if args.precision == "fp64" else qmin.QM_FP32, args.precision)
+        return
     from bench import build_workload, fire_params
     precision = qmin.QM_FP64 if args.precision == "fp64" else qmin.QM_FP32
     lat, force = build_workload(qmin, args.size, precision)

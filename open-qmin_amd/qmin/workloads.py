@@ -1,5 +1,6 @@
-"""Seeded synthetic workloads: the ensemble of uneven lattices and helpers
-shared by bench.py, the warm-start data pipeline and the tests.
+"""Seeded synthetic workloads: the ensemble of uneven lattices, the
+full-elasticity lattice, and helpers shared by bench.py, the profiling runs,
+the warm-start data pipeline and the tests.
 
 Ensemble: lattices whose edge triples are drawn uniformly from the 65 triples
 of {24, 32, 48, 64, 96}^3 with max/min <= 2, each with 0-3 spheres (radius 4-16,
@@ -51,3 +52,22 @@ def build_lattice(qmin, spec, precision, S0):
     for centre, r, btype, W in spec["spheres"]:
         lat.sphere(centre, r, btype, W, S0)
     return lat
+
+
+def full_elasticity(qmin, L, precision, seed=1234, geometry_seed=384):
+    """The full-elasticity workload on an L^3 periodic lattice: L1..L4 and L6 =
+    (2.32, 1.0, 0.5, 0.2, 0.3) (open-Qmin has no L5 term), a uniform E field
+    (0, 0, 0.5) with delta eps = 1 (eps = eps0 = 1), and four homeotropic
+    spheres of radius 12-24 (scaled by L/384) with W uniform in [1, 10] at
+    seeded uniform positions; seeded random director start.  Returns
+    (lattice, force); the reference GPU metric semantics."""
+    A, B, C, S0 = phase_constants()
+    rng = np.random.default_rng(geometry_seed)
+    lat = qmin.Lattice(L, L, L, precision=precision)
+    lat.random_init(seed, S0)
+    for _ in range(4):
+        c = tuple(float(v) for v in rng.uniform(0, L, 3))
+        lat.sphere(c, float(rng.integers(12, 25)) * L / 384, qmin.HOMEOTROPIC, float(rng.uniform(1, 10)), S0)
+    f = qmin.LandauDeGennesLC(A, B, C, 2.32, 1.0, 0.5, 0.2, 0.3, semantics=qmin.SEMANTICS_REF_GPU)
+    f.set_efield((0.0, 0.0, 0.5), 1.0, 1.0, 1.0)
+    return lat, f

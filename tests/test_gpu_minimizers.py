@@ -114,3 +114,24 @@ def test_fp32_minimizers_track_fp64():
         m.set_trace(32)
         m.minimize(f, g)
         np.testing.assert_allclose(m.trace(), tr_ref, rtol=1e-4, err_msg=name)
+
+
+@pytest.mark.parametrize("kind", ["adam", "nesterov", "lolbfgs"])
+def test_minimizer_moved_to_a_new_lattice_restarts(kind):
+    """A minimizer used on one lattice and then on a new lattice of the same
+    size (possibly at the same address once the first is destroyed) starts over
+    on the new lattice: iteration 0, fresh history, as a new minimizer does."""
+    make = {"adam": lambda: qmin.Adam(dt=1e-3, max_iter=10),
+            "nesterov": lambda: qmin.Nesterov(dt=1e-2, mu=0.5, scheduled=True, max_iter=10),
+            "lolbfgs": lambda: qmin.LoLBFGS(m=5, dt=1e-2, max_iter=10)}[kind]
+    o, g, f = build_pair((16, 14, 12), MULTI, 0, True)
+    m = make()
+    m.minimize(f, g)
+    del g
+    _, g2, _ = build_pair((16, 14, 12), MULTI, 0, True, seed=99)
+    _, g3, _ = build_pair((16, 14, 12), MULTI, 0, True, seed=99)
+    it, fm = m.minimize(f, g2)
+    it_ref, fm_ref = make().minimize(f, g3)
+    assert it == it_ref == 10
+    assert fm == fm_ref
+    assert np.array_equal(g2.q(), g3.q())
