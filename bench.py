@@ -146,9 +146,10 @@ def compare_modes(qmin, L, fc, switch_fc=None):
 
 def ensemble_throughput(qmin, count=1024, fc=1e-7, max_iter=10000, sequential_sample=32, graph_iterations=32):
     """The ensemble workload (seeded uneven lattices with spheres, L1 only, FP32
-    state with FP64 reductions): FIRE on all lattices at once, batched launches
-    replayed from a hipGraph; the first lattices are also minimized one by one
-    for comparison.  dt_max is 40 dt: at the driver's 100 dt FIRE diverges on
+    state with FP64 reductions): the tile autotuner picks the batched pass's
+    shape (outside the timed region), then FIRE on all lattices at once, batched
+    launches replayed from a hipGraph; the first lattices are also minimized one
+    by one for comparison.  dt_max is 40 dt: at the driver's 100 dt FIRE diverges on
     stiff planar-anchored spheres (the reference algorithm does too)."""
     from qmin import workloads
     A, B, C, S0 = workloads.phase_constants()
@@ -160,6 +161,8 @@ def ensemble_throughput(qmin, count=1024, fc=1e-7, max_iter=10000, sequential_sa
     force = qmin.LandauDeGennesLC(A, B, C, 2.32, semantics=qmin.SEMANTICS_REF_GPU)
     params = dict(dt=5e-4, dt_max=2e-2, fc=fc, max_iter=max_iter)
     ens = qmin.Ensemble(lats)
+    ta = time.perf_counter()
+    tile, zchunk, pass_ms = ens.autotune(force)
     t1 = time.perf_counter()
     its, fms = ens.fire_minimize(qmin.FIRE(**params), force, graph_iterations)
     t2 = time.perf_counter()
@@ -173,6 +176,8 @@ def ensemble_throughput(qmin, count=1024, fc=1e-7, max_iter=10000, sequential_sa
     return {"lattices": count, "sites": int(sum(sites)), "dtype": "f32 (FP64 reductions)",
             "forceMax_below": fc, "max_iterations": max_iter, "seconds": t2 - t1, "build_seconds": build_s,
             "site_updates_per_s": updates / (t2 - t1), "converged": int(sum(1 for f in fms if f <= fc)),
+            "autotune": {"tile": ["32x8", "64x4", "32x16"][tile], "zchunk": zchunk, "pass_ms": pass_ms,
+                         "seconds": t1 - ta},
             "iterations_min_mean_max": [int(min(its)), float(sum(its)) / count, int(max(its))],
             "one_by_one": {"lattices": sequential_sample, "site_updates_per_s": seq_updates / (t4 - t3),
                            "same_iterations": [int(i) for i in its[:sequential_sample]] == seq_its}}

@@ -239,6 +239,12 @@ int qm_ensemble_destroy(qm_ensemble* e);
  * (arrays of count, may be NULL) */
 int qm_ensemble_fire_minimize(qm_ensemble* e, const qm_fire* params, qm_ldg* f, int graphIterations,
                               int* iterations, double* forceMax);
+/* tile autotuner of the batched pass, the role of kernelTuner
+ * (src/utilities/kernelTuner.cpp:5-125) for ensembles: times the batched fused
+ * FIRE pass with HIP events for every tile shape and z chunks {8, 16, 32, 64} on
+ * the lattices' current states (which it leaves unchanged) and keeps the
+ * fastest for later qm_ensemble_fire_minimize calls (outputs may be NULL). */
+int qm_ensemble_autotune(qm_ensemble* e, qm_ldg* f, int* tile, int* zchunk, double* ms);
 
 /* ---- learned warm start -----------------------------------------------------
  * No reference counterpart.  Building blocks of a small 3-D convolutional

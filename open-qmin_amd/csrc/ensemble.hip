@@ -36,6 +36,9 @@ struct qm_ensemble {
     int2* dBoundaryMap = nullptr;
     EnsembleStatus* dStatus = nullptr;
     int marchBlocks = 0, boundaryBlocks = 0;
+    int tile = -1;     // tile shape of the batched pass (-1: the first lattice's default)
+    int zchunk = 0;    // z chunk (0: each lattice's own setting, else whole columns up to 32 planes)
+    int usedTile = 0;  // the shape the current descriptors were built for
     hipGraphExec_t graph = nullptr;
     int graphIters = 0;
 
@@ -69,7 +72,8 @@ static std::vector<int2> xcd_order(const std::vector<int2>& logical) {
 template <typename R>
 static int build_descs(qm_ensemble* E, const qm_ldg* f, const qm_fire* params) {
     const int count = (int)E->lats.size();
-    const int tile = active_tile(E->lats[0]);
+    const int tile = E->tile >= 0 ? E->tile : active_tile(E->lats[0]);
+    E->usedTile = tile;
     std::vector<MarchArgs<R>> md(count);
     std::vector<BoundaryArgs<R>> bd(count);
     std::vector<FinalizeDesc> fd(count);
@@ -78,7 +82,7 @@ static int build_descs(qm_ensemble* E, const qm_ldg* f, const qm_fire* params) {
         qm_lattice* L = E->lats[k];
         L->tileOverride = tile;
         // whole columns (or chunks of 32 planes): an ensemble has work-groups enough
-        L->zchunk = L->zchunkOverride > 0 ? L->zchunkOverride : std::min(L->lz, 32);
+        L->zchunk = E->zchunk > 0 ? E->zchunk : L->zchunkOverride > 0 ? L->zchunkOverride : std::min(L->lz, 32);
         FireConst fc = fire_const(params, L);
         fill_args<R>(L, f, md[k], bd[k], fc);
         md[k].gate = 0;
@@ -142,7 +146,7 @@ static int ensemble_fire(qm_ensemble* E, const qm_fire* params, qm_ldg* f, int g
         if (int rc = push_ctrl(L)) return rc;
     }
     if (int rc = build_descs<R>(E, f, params)) return rc;
-    const int terms = terms_of(f), tile = active_tile(E->lats[0]);
+    const int terms = terms_of(f), tile = E->usedTile;
     // sim->computeForces() at the start of minimize, on every lattice at once
     if (int rc = enqueue_pass<R>(E, kForce, terms, tile)) return rc;
     // capture graphIters FIRE iterations once, replay until every lattice is done
@@ -182,6 +186,75 @@ static int ensemble_fire(qm_ensemble* E, const qm_fire* params, qm_ldg* f, int g
     return QM_OK;
 }
 
+// The tile autotuner for ensembles (the role of kernelTuner,
+// src/utilities/kernelTuner.cpp:5-125, for the batched pass): times the batched
+// fused FIRE pass for every tile shape and z chunks {8, 16, 32, 64} with HIP
+// events and keeps the fastest for later fire_minimize calls.  The timed passes
+// read each lattice's current state set and write only the other set and the
+// partial sums, which the next real pass overwrites; the control blocks are
+// restored, so the lattices are left as they were.
+template <typename R>
+static int ensemble_autotune(qm_ensemble* E, qm_ldg* f, int* tileOut, int* zchunkOut, double* msOut) {
+    const int count = (int)E->lats.size();
+    std::vector<Ctrl> saved(count);
+    for (int k = 0; k < count; ++k) {
+        qm_lattice* L = E->lats[k];
+        if (int rc = prepare(L, f)) return rc;
+        if (int rc = sync_ctrl_to_host(L)) return rc;
+        saved[k] = L->hCtrl;
+        L->hCtrl.done = 0;  // a done lattice would skip its work-groups
+        if (int rc = push_ctrl(L)) return rc;
+    }
+    const qm_fire defaults{};  // FIRE constants of the descriptors; the timed kernel does not read them
+    const int terms = terms_of(f);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    const int savedTile = E->tile, savedZ = E->zchunk;
+    double best = 1e300;
+    int bestTile = 0, bestZ = 0, rc = QM_OK;
+    const int reps = 3;
+    for (int tile = 0; tile < kNumTiles && rc == QM_OK; ++tile) {
+        for (int zc : {8, 16, 32, 64}) {
+            E->tile = tile;
+            E->zchunk = zc;
+            if ((rc = build_descs<R>(E, f, &defaults))) break;
+            const auto* md = static_cast<const MarchArgs<R>*>(E->dMarchDescs);
+            hipError_t err = launch_march_batch<R>(md, E->dMarchMap, E->marchBlocks, kFire, terms, tile, E->stream);
+            if (err == hipSuccess) err = hipEventRecord(e0, E->stream);
+            for (int r = 0; r < reps && err == hipSuccess; ++r)
+                err = launch_march_batch<R>(md, E->dMarchMap, E->marchBlocks, kFire, terms, tile, E->stream);
+            if (err == hipSuccess) err = hipEventRecord(e1, E->stream);
+            if (err == hipSuccess) err = hipEventSynchronize(e1);
+            float t = 0;
+            if (err == hipSuccess) err = hipEventElapsedTime(&t, e0, e1);
+            if (err != hipSuccess) {
+                rc = fail(QM_EHIP, std::string("HIP error: ") + hipGetErrorString(err));
+                break;
+            }
+            if (t / reps < best) {
+                best = t / reps;
+                bestTile = tile;
+                bestZ = zc;
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    E->tile = rc == QM_OK ? bestTile : savedTile;
+    E->zchunk = rc == QM_OK ? bestZ : savedZ;
+    E->release_device();
+    for (int k = 0; k < count; ++k) {
+        E->lats[k]->hCtrl = saved[k];
+        if (int r2 = push_ctrl(E->lats[k])) return r2;
+    }
+    if (rc) return rc;
+    if (tileOut) *tileOut = bestTile;
+    if (zchunkOut) *zchunkOut = bestZ;
+    if (msOut) *msOut = best;
+    return QM_OK;
+}
+
 extern "C" {
 
 int qm_ensemble_create(qm_lattice* const* lattices, int count, qm_ensemble** out) {
@@ -208,6 +281,14 @@ int qm_ensemble_destroy(qm_ensemble* E) {
     (void)hipStreamDestroy(E->stream);
     delete E;
     return QM_OK;
+}
+
+int qm_ensemble_autotune(qm_ensemble* E, qm_ldg* f, int* tile, int* zchunk, double* ms) {
+    if (!E || !f) return fail(QM_EINVAL, "bad argument");
+    for (qm_lattice* L : E->lats)
+        if (L->precision != E->precision) return fail(QM_ESTATE, "a lattice changed precision after joining");
+    return E->precision == QM_FP64 ? ensemble_autotune<double>(E, f, tile, zchunk, ms)
+                                   : ensemble_autotune<float>(E, f, tile, zchunk, ms);
 }
 
 int qm_ensemble_fire_minimize(qm_ensemble* E, const qm_fire* params, qm_ldg* f, int graphIterations, int* iterations,

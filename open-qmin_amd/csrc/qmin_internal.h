@@ -71,7 +71,6 @@ struct qm_fire {
     int converged = 0, nan = 0;
     int traceCap = 0;
     std::vector<double> trace;
-    qm_lattice* bound = nullptr;   // lattice holding this minimizer's device scalars
 };
 
 

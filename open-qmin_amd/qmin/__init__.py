@@ -61,6 +61,7 @@ _API = {
     "qm_ensemble_create": [ctypes.POINTER(_VP), _i, ctypes.POINTER(_VP)],
     "qm_ensemble_destroy": [_VP],
     "qm_ensemble_fire_minimize": [_VP, _VP, _VP, _i, ctypes.POINTER(ctypes.c_int), _D],
+    "qm_ensemble_autotune": [_VP, _VP, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), _D],
     "qm_lattice_autotune": [_VP, _VP, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), _D],
     "qm_lattice_set_precision": [_VP, _i],
     "qm_lattice_set_field": [_VP, _i, _D],
@@ -504,6 +505,14 @@ class Ensemble:
         fm = np.zeros(n)
         _check(self.lib.qm_ensemble_fire_minimize(self.h, fire.h, force.h, graph_iterations, it, _dp(fm)))
         return np.array(list(it)), fm
+
+    def autotune(self, force):
+        """Time the batched pass for every tile shape and z chunk on the current
+        states (left unchanged) and keep the fastest; returns (tile, zchunk, ms)."""
+        tile, zc, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _check(self.lib.qm_ensemble_autotune(self.h, force.h, ctypes.byref(tile), ctypes.byref(zc),
+                                             ctypes.byref(ms)))
+        return tile.value, zc.value, ms.value
 
 
 def timing_enable(on=True):

@@ -76,3 +76,27 @@ def test_ensemble_fp32_tracks_single_lattice_fp32():
         st = qmin.FIRE(fc=1e-7, max_iter=120).minimize(force, single)
         assert fm == pytest.approx(st.forceMax, rel=1e-4)
         np.testing.assert_allclose(lat.q(), single.q(), atol=1e-5)
+
+
+def test_ensemble_autotune_leaves_states_and_results_unchanged():
+    """The ensemble tile autotuner leaves every lattice as it was, and FIRE on
+    the tuned shape stops where each lattice's own minimize() does (per-site
+    forces do not depend on the tile; the reductions only by rounding)."""
+    lats = [make(s, True) for s in SPECS]
+    force = qmin.LandauDeGennesLC(A, B, C, 2.32, 0.5, 0.3, semantics=1)
+    q0 = [l.q() for l in lats]
+    ens = qmin.Ensemble(lats)
+    tile, zchunk, ms = ens.autotune(force)
+    assert tile in (0, 1, 2) and zchunk in (8, 16, 32, 64) and ms > 0
+    for l, q in zip(lats, q0):
+        # equal_nan: the object site at an integer sphere centre has no direction
+        # (0/0 in the reference's Q^B as well); it never enters a force
+        assert np.array_equal(l.q(), q, equal_nan=True)
+    its, fms = ens.fire_minimize(qmin.FIRE(fc=1e-6, max_iter=150), force, 16)
+    for spec, lat, it, fm in zip(SPECS, lats, its, fms):
+        o = make(spec, False)
+        o.set_params(A, B, C, 2.32, 0.5, 0.3, semantics=1)
+        it_ref, tr = o.fire(fc=1e-6, max_iter=150)
+        assert it == it_ref, spec
+        assert fm == pytest.approx(tr[-1, 0], rel=1e-9), spec
+        np.testing.assert_allclose(lat.q(), o.q(), rtol=0, atol=1e-10, err_msg=str(spec))
