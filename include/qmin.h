@@ -83,8 +83,19 @@ int qm_lattice_get_field(qm_lattice* lat, int which, double* aos);
 /* qTensorLatticeModel::setNematicQTensorRandomly after noiseSource::setReproducibleSeed
  * (src/model/qTensorLatticeModel.cpp:134-177, src/utilities/noiseSource.cpp:38-47) */
 int qm_lattice_random_init(qm_lattice* lat, int seed, double S0);
-/* multirankQTensorLatticeModel::setUniformDirectors-like uniform texture from a director */
+/* multirankQTensorLatticeModel::setUniformDirectors (multirankQTensorLatticeModel.cpp:79-88;
+ * openQmin -z 2): qTensorFromDirector of one director on every site with t <= 0 */
 int qm_lattice_uniform_init(qm_lattice* lat, double nx, double ny, double nz, double S0);
+/* setNematicQTensorRandomly / setRandomDirectors with globallyAligned = true
+ * (qTensorLatticeModel.cpp:134-177, multirankQTensorLatticeModel.cpp:52-77;
+ * openQmin -z 1): one director from the first two draws of mt19937(seed)
+ * (theta = acos(2u - 1), phi = 2 pi u) on every site with t <= 0 */
+int qm_lattice_random_uniform_init(qm_lattice* lat, int seed, double S0);
+/* multirankQTensorLatticeModel::setDirectorFromFunction
+ * (multirankQTensorLatticeModel.cpp:90-110; openQmin -z 3): directors holds the
+ * function's (nx, ny, nz, S0) for every site in site order (4 n doubles);
+ * qTensorFromDirector is applied on the sites with t <= 0 */
+int qm_lattice_director_init(qm_lattice* lat, const double* directors);
 /* multirankSimulation::createMultirankBoundaryObject + cubicLattice::createBoundaryObject
  * (src/simulation/multirankSimulationBoundaries.cpp:512-543, src/model/cubicLattice.cpp:267-318);
  * xyz is [n][3] (wrapped periodically), qb is [n][5]; returns the object index via *obj */

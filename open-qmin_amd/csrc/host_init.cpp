@@ -53,6 +53,34 @@ void random_nematic(long long n, const int* types, int seed, double S0, double* 
     }
 }
 
+// The globally aligned case of the same functions (and of
+// multirankQTensorLatticeModel::setRandomDirectors, multirankQTensorLatticeModel.cpp:52-77):
+// the director of the two global draws of the reproducible stream.
+void random_director(int seed, double* n) {
+    const double pi = 3.14159265358979323846;
+    std::mt19937 gen(seed);
+    std::uniform_real_distribution<double> d(0.0, 1.0);
+    const double theta = std::acos(2.0 * d(gen) - 1);
+    const double phi = 2.0 * pi * d(gen);
+    n[0] = std::cos(phi) * std::sin(theta);
+    n[1] = std::sin(phi) * std::sin(theta);
+    n[2] = std::cos(theta);
+}
+
+// normalizeDirector (inc/functions.h:193-205): unit vector, x for a zero vector
+void normalize_director(double* n) {
+    const double a = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    if (a == 0) {
+        n[0] = 1;
+        n[1] = 0;
+        n[2] = 0;
+    } else {
+        n[0] = n[0] / a;
+        n[1] = n[1] / a;
+        n[2] = n[2] / a;
+    }
+}
+
 // multirankSimulation::createSphericalColloid (src/simulation/multirankSimulationBoundaries.cpp:169-215):
 // sites with |r - c|^2 < R^2 over the half-open box [ceil(c - R), floor(c + R))
 void sphere_sites(double cx, double cy, double cz, double radius, int btype, double S0, std::vector<int>& xyz,

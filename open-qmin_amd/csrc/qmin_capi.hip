@@ -385,6 +385,28 @@ int qm_lattice_uniform_init(qm_lattice* L, double nx, double ny, double nz, doub
     return upload_field(L, QM_FIELD_Q, q.data());
 }
 
+// setNematicQTensorRandomly / setRandomDirectors with globallyAligned = true
+int qm_lattice_random_uniform_init(qm_lattice* L, int seed, double S0) {
+    if (!L) return fail(QM_EINVAL, "null lattice");
+    double n[3];
+    random_director(seed, n);
+    return qm_lattice_uniform_init(L, n[0], n[1], n[2], S0);
+}
+
+// multirankQTensorLatticeModel::setDirectorFromFunction (multirankQTensorLatticeModel.cpp:90-110):
+// directors holds (nx, ny, nz, S0) per site in site order, the function's values
+int qm_lattice_director_init(qm_lattice* L, const double* directors) {
+    if (!L || !directors) return fail(QM_EINVAL, "bad argument");
+    std::vector<double> q(5 * L->n);
+    if (int rc = download_field(L, QM_FIELD_Q, q.data())) return rc;
+    for (long long i = 0; i < L->n; ++i)
+        if (L->types[i] <= 0) {
+            const double* d = directors + 4 * i;
+            director_q(d[0], d[1], d[2], d[3], &q[5 * i]);
+        }
+    return upload_field(L, QM_FIELD_Q, q.data());
+}
+
 // createMultirankBoundaryObject (multirankSimulationBoundaries.cpp:512-543) then
 // cubicLattice::createBoundaryObject (cubicLattice.cpp:267-318): write Q^B,
 // mark the sites j, mark their non-object neighbours -1.

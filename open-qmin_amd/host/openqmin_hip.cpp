@@ -6,7 +6,8 @@
 //   -a/-b/-c phase constants, --L1 --L2 --L3 --L4 --L6, --randomSeed,
 //   --boundaryFile, --saveFile, --initialConfigurationFile, --spatiallyVaryingFieldFile,
 //   --hField{X,Y,Z,Mu0,Chi,DeltaChi},
-//   --eField{X,Y,Z,Epsilon0,Epsilon,DeltaEpsilon}, -z/--initializationSwitch (0 only), -v.
+//   --eField{X,Y,Z,Epsilon0,Epsilon,DeltaEpsilon}, -z/--initializationSwitch (-1, 0, 1, 2, 3 as
+//   setInitialConditions.h:21-99, with its example director function for 3), -v.
 // Additions of this build (no reference flag):
 //   --minimizer fire|adam|nesterov|lolbfgs   (default fire, as openQmin)
 //   --precision fp64|fp32|mixed              (mixed = FP32 to the tolerance, then FP64 polish)
@@ -91,8 +92,11 @@ Args parse(int argc, char** argv) {
         else
             a.v[key] = val;
     }
-    if (a.has("initializationSwitch") && a.i("initializationSwitch", 0) != 0)
-        throw std::runtime_error("only the random initial state (-z 0) is provided");
+    if (a.has("initializationSwitch") && (a.i("initializationSwitch", 0) < -1 || a.i("initializationSwitch", 0) > 3))
+        throw std::runtime_error("-z is -1 (file), 0 (random), 1 (random uniform), 2 (fixed uniform) or 3 (function)");
+    if (a.has("initializationSwitch") && a.i("initializationSwitch", 0) == -1 &&
+        a.s("initialConfigurationFile", "NONE") == "NONE")
+        throw std::runtime_error("-z -1 needs --initialConfigurationFile");
     if (a.has("precision") && a.v.at("precision") != "fp64" && a.v.at("precision") != "fp32" &&
         a.v.at("precision") != "mixed")
         throw std::runtime_error("--precision is fp64, fp32 or mixed");
@@ -180,11 +184,31 @@ int main(int argc, char** argv) {
         sim->addUpdater(upd, Configuration);
 
         const scalar S0 = (-B + std::sqrt(B * B - 24 * A * C)) / (6 * C);
+        // setInitialConditions.h:21-99
         const std::string init = a.s("initialConfigurationFile", "NONE");
-        if (init != "NONE")
+        const int initializationSwitch = init != "NONE" ? -1 : a.i("initializationSwitch", 0);
+        if (initializationSwitch == -1) {
             sim->loadState(init);
-        else
+        } else if (initializationSwitch == 0) {
             Configuration->setNematicQTensorRandomly(noise, S0);
+        } else if (initializationSwitch == 1) {
+            // one random director for every site, from its own reproducible stream
+            noiseSource multirankNoise(true);
+            multirankNoise.setReproducibleSeed(randomSeed == -1 ? 13371 : randomSeed);
+            Configuration->setRandomDirectors(multirankNoise, S0, true);
+        } else if (initializationSwitch == 2) {
+            scalar3 targetDirector = make_scalar3(1. / 2, 0, 1 / 2.);
+            normalizeDirector(targetDirector);
+            Configuration->setUniformDirectors(targetDirector, S0);
+        } else {
+            // the example function of setInitialConditions.h
+            Configuration->setDirectorFromFunction([](scalar x, scalar y, scalar z) {
+                scalar3 director = make_scalar3(x, x + y + std::cos(z), 0);
+                const scalar s0 = 0.53 + 0.0000000001 * (x + y + z);
+                normalizeDirector(director);
+                return scalar4{director.x, director.y, director.z, s0};
+            });
+        }
 
         if (a.s("boundaryFile", "NONE") != "NONE") sim->createBoundaryFromFile(a.s("boundaryFile", ""), verbose);
         for (const std::string& w : a.walls) {

@@ -33,7 +33,24 @@ typedef double scalar;
 struct scalar3 {
     double x, y, z;
 };
+struct scalar4 {
+    double x, y, z, w;
+};
 inline scalar3 make_scalar3(double x, double y, double z) { return scalar3{x, y, z}; }
+
+// normalizeDirector (inc/functions.h:193-205): unit vector, x for a zero vector
+inline void normalizeDirector(scalar3& p) {
+    const scalar a = std::sqrt(p.x * p.x + p.y * p.y + p.z * p.z);
+    if (a == 0) {
+        p.x = 1;
+        p.y = 0;
+        p.z = 0;
+    } else {
+        p.x = p.x / a;
+        p.y = p.y / a;
+        p.z = p.z / a;
+    }
+}
 
 // src/utilities/std_include.h (distortionEnergyType) and src/model/latticeBoundaries.h:8-40
 enum class distortionEnergyType { oneConstant, multiConstant };
@@ -106,12 +123,37 @@ class qTensorLatticeModel : public simpleModel {
 
     // qTensorLatticeModel::setNematicQTensorRandomly (qTensorLatticeModel.cpp:134-177)
     void setNematicQTensorRandomly(noiseSource& noise, scalar s0, bool globallyAligned = false) {
-        if (globallyAligned) throw qminError(QM_EINVAL, "globallyAligned initialisation is not provided");
-        qmin_host::check(qm_lattice_random_init(lat_, noise.seedForDraw(), s0), "setNematicQTensorRandomly");
+        if (globallyAligned)
+            qmin_host::check(qm_lattice_random_uniform_init(lat_, noise.seedForDraw(), s0), "setNematicQTensorRandomly");
+        else
+            qmin_host::check(qm_lattice_random_init(lat_, noise.seedForDraw(), s0), "setNematicQTensorRandomly");
     }
-    // multirankQTensorLatticeModel::setUniformDirectors
+    // multirankQTensorLatticeModel::setRandomDirectors (multirankQTensorLatticeModel.cpp:52-77): the same
+    // draws as setNematicQTensorRandomly
+    void setRandomDirectors(noiseSource& noise, scalar s0, bool globallyAligned = false) {
+        setNematicQTensorRandomly(noise, s0, globallyAligned);
+    }
+    // multirankQTensorLatticeModel::setUniformDirectors (multirankQTensorLatticeModel.cpp:79-88)
     void setUniformDirectors(scalar3 d, scalar s0) {
         qmin_host::check(qm_lattice_uniform_init(lat_, d.x, d.y, d.z, s0), "setUniformDirectors");
+    }
+    // multirankQTensorLatticeModel::setDirectorFromFunction (multirankQTensorLatticeModel.cpp:90-110):
+    // func(x, y, z) gives the director and S0 at a site; it is evaluated at every site (the values of
+    // object sites are not used)
+    template <typename F>
+    void setDirectorFromFunction(F&& func) {
+        std::vector<double> d(4 * (size_t)N);
+        size_t k = 0;
+        for (int z = 0; z < latticeSites.z; ++z)
+            for (int y = 0; y < latticeSites.y; ++y)
+                for (int x = 0; x < latticeSites.x; ++x, k += 4) {
+                    const scalar4 r = func(scalar(x), scalar(y), scalar(z));
+                    d[k] = r.x;
+                    d[k + 1] = r.y;
+                    d[k + 2] = r.z;
+                    d[k + 3] = r.w;
+                }
+        qmin_host::check(qm_lattice_director_init(lat_, d.data()), "setDirectorFromFunction");
     }
     // qTensorLatticeModel::computeDefectMeasures (qTensorLatticeModel.cpp:95-133)
     void computeDefectMeasures(int defectType) {

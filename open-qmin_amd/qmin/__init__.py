@@ -68,6 +68,8 @@ _API = {
     "qm_lattice_get_field": [_VP, _i, _D],
     "qm_lattice_random_init": [_VP, _i, _d],
     "qm_lattice_uniform_init": [_VP, _d, _d, _d, _d],
+    "qm_lattice_random_uniform_init": [_VP, _i, _d],
+    "qm_lattice_director_init": [_VP, _D],
     "qm_lattice_add_object": [_VP, _I, ctypes.c_int64, _D, _i, _d, _d, ctypes.POINTER(ctypes.c_int)],
     "qm_lattice_wall": [_VP, _i, _i, _i, _d, _d, ctypes.POINTER(ctypes.c_int)],
     "qm_lattice_sphere": [_VP, _d, _d, _d, _d, _i, _d, _d, ctypes.POINTER(ctypes.c_int)],
@@ -241,6 +243,25 @@ class Lattice:
 
     def uniform_init(self, director, S0):
         _check(self.lib.qm_lattice_uniform_init(self.h, director[0], director[1], director[2], S0))
+
+    def random_uniform_init(self, seed, S0):
+        """setNematicQTensorRandomly(noise, S0, globallyAligned=True): one director,
+        from the first two draws of mt19937(seed), on every site with t <= 0."""
+        _check(self.lib.qm_lattice_random_uniform_init(self.h, seed, S0))
+
+    def director_init(self, directors):
+        """setDirectorFromFunction: directors is an (n, 4) array of (nx, ny, nz, S0)
+        in site order, or a function f(x, y, z) of integer coordinate arrays (each
+        of n sites, x fastest) returning those four arrays."""
+        if callable(directors):
+            z, y, x = (c.ravel() for c in np.meshgrid(np.arange(self.shape[2]), np.arange(self.shape[1]),
+                                                      np.arange(self.shape[0]), indexing="ij"))
+            directors = np.stack([np.broadcast_to(np.asarray(a, np.float64), x.shape) for a in directors(x, y, z)],
+                                 axis=1)
+        d = np.ascontiguousarray(directors, np.float64)
+        if d.shape != (self.n, 4):
+            raise ValueError("directors must have shape (n, 4)")
+        _check(self.lib.qm_lattice_director_init(self.h, _dp(d)))
 
     def add_object(self, xyz, qb, btype, W, S0):
         xyz = np.ascontiguousarray(xyz, np.int32)

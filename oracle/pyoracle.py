@@ -70,6 +70,8 @@ def oracle_lib():
         for n in ("orc_get_q", "orc_set_q", "orc_get_v", "orc_get_f"):
             _sig(lib, n, None, [_VP, _D])
         _sig(lib, "orc_random_init", None, [_VP, _i, _d])
+        _sig(lib, "orc_random_uniform_init", None, [_VP, _i, _d])
+        _sig(lib, "orc_director_init", None, [_VP, _D])
         _sig(lib, "orc_add_object", _i, [_VP, _I, _i, _D, _i, _d, _d])
         _sig(lib, "orc_wall", _i, [_VP, _i, _i, _i, _d, _d])
         _sig(lib, "orc_sphere", _i, [_VP, _d, _d, _d, _d, _i, _d, _d])
@@ -119,6 +121,8 @@ def ref_lib():
         _sig(lib, "qmref_model_destroy", None, [_VP])
         _sig(lib, "qmref_model_neighbors", None, [_VP, _I])
         _sig(lib, "qmref_model_random_init", None, [_VP, _i, _d])
+        _sig(lib, "qmref_model_random_init_aligned", None, [_VP, _i, _d])
+        _sig(lib, "qmref_normalize_director", None, [_D])
         _sig(lib, "qmref_model_add_object", None, [_VP, _I, _i, _i, _d, _d])
         _sig(lib, "qmref_model_get_q", None, [_VP, _D])
         _sig(lib, "qmref_model_set_q", None, [_VP, _D])
@@ -216,6 +220,14 @@ class OracleSystem:
 
     def random_init(self, seed, S0):
         self.lib.orc_random_init(self.h, seed, S0)
+
+    def random_uniform_init(self, seed, S0):
+        self.lib.orc_random_uniform_init(self.h, seed, S0)
+
+    def director_init(self, n4):
+        n4 = np.ascontiguousarray(n4, np.float64)
+        assert n4.shape == (self.n, 4)
+        self.lib.orc_director_init(self.h, dptr(n4))
 
     def add_object(self, xyz, qb, btype, W, S0):
         xyz = np.ascontiguousarray(xyz, np.int32)
@@ -321,6 +333,9 @@ class RefModel:
 
     def random_init(self, seed, S0):
         self.lib.qmref_model_random_init(self.h, seed, S0)
+
+    def random_init_aligned(self, seed, S0):
+        self.lib.qmref_model_random_init_aligned(self.h, seed, S0)
 
     def add_object(self, sites, btype, P1, P2):
         sites = np.ascontiguousarray(sites, np.int32)

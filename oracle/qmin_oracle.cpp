@@ -586,6 +586,30 @@ void orc_random_init(orc_system* s, int seed, double S0) {
     }
 }
 
+// setNematicQTensorRandomly(noise, S0, globallyAligned = true)
+// (qTensorLatticeModel.cpp:134-177; multirankQTensorLatticeModel.cpp:52-77 draws the
+// same way): every t <= 0 site gets the director of the two global draws; the
+// per-site draws are made and discarded, which does not change the result.
+void orc_random_uniform_init(orc_system* s, int seed, double S0) {
+    std::mt19937 gen(seed);
+    std::uniform_real_distribution<double> d(0.0, 1.0);
+    const double theta = std::acos(2.0 * d(gen) - 1);
+    const double phi = 2.0 * kPi * d(gen);
+    const double nx = std::cos(phi) * std::sin(theta), ny = std::sin(phi) * std::sin(theta), nz = std::cos(theta);
+    for (int64_t pp = 0; pp < s->n; ++pp)
+        if (s->types[pp] <= 0) qFromDirector(nx, ny, nz, S0, &s->q[5 * pp]);
+}
+
+// multirankQTensorLatticeModel::setDirectorFromFunction (multirankQTensorLatticeModel.cpp:90-110)
+// with the function's values given: n4 = (nx, ny, nz, S0) per site, site order
+void orc_director_init(orc_system* s, const double* n4) {
+    for (int64_t pp = 0; pp < s->n; ++pp)
+        if (s->types[pp] <= 0) {
+            const double* d = n4 + 4 * pp;
+            qFromDirector(d[0], d[1], d[2], d[3], &s->q[5 * pp]);
+        }
+}
+
 // multirankSimulation::createMultirankBoundaryObject (multirankSimulationBoundaries.cpp:512-543)
 // followed by cubicLattice::createBoundaryObject (cubicLattice.cpp:267-318).
 int orc_add_object(orc_system* s, const int32_t* xyz, int n, const double* qb, int btype, double W, double S0) {

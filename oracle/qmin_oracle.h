@@ -33,6 +33,8 @@ void orc_set_types(orc_system* s, const int32_t* in);
 void orc_get_q(const orc_system* s, double* out);
 void orc_set_q(orc_system* s, const double* in);
 void orc_random_init(orc_system* s, int seed, double S0);
+void orc_random_uniform_init(orc_system* s, int seed, double S0);
+void orc_director_init(orc_system* s, const double* n4);
 // btype 0 homeotropic, 1 degenerate planar; sites are (x,y,z) triples, qb is [n][5]
 int orc_add_object(orc_system* s, const int32_t* xyz, int n, const double* qb, int btype, double W, double S0);
 int orc_wall(orc_system* s, int xyz, int plane, int btype, double W, double S0);

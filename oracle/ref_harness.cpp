@@ -251,6 +251,25 @@ void qmref_model_random_init(void* h, int seed, double S0) {
     m->setNematicQTensorRandomly(noise, S0);
 }
 
+void qmref_model_random_init_aligned(void* h, int seed, double S0) {
+    auto& m = static_cast<RefModel*>(h)->model;
+    noiseSource noise(true);
+    noise.setReproducibleSeed(seed);
+    m->setNematicQTensorRandomly(noise, S0, true);
+}
+
+// normalizeDirector (inc/functions.h), in place on n[3]
+void qmref_normalize_director(double* n) {
+    scalar3 v;
+    v.x = n[0];
+    v.y = n[1];
+    v.z = n[2];
+    normalizeDirector(v);
+    n[0] = v.x;
+    n[1] = v.y;
+    n[2] = v.z;
+}
+
 // sites: lattice indices; btype as in qmref_anchoring_force
 void qmref_model_add_object(void* h, const int* sites, int n, int btype, double P1, double P2) {
     auto& m = static_cast<RefModel*>(h)->model;

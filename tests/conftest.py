@@ -26,3 +26,28 @@ def phase():
     import pyoracle as po
     A, B, C = po.rescaled_phase()
     return A, B, C, po.uniaxial_s0(A, B, C)
+
+
+def example_directors(shape):
+    """(nx, ny, nz, S0) per site, in site order, of the example director function
+    in the reference's setInitialConditions.h (openQmin -z 3): director
+    (x, x + y + cos z, 0) normalized as normalizeDirector does (inc/functions.h:
+    193-205), S0 = 0.53 + 1e-10 (x + y + z).  math.cos is the C library's cos,
+    as in the reference's and the driver's builds."""
+    import math
+    import numpy as np
+    lx, ly, lz = shape
+    out = np.zeros((lx * ly * lz, 4))
+    i = 0
+    for z in range(lz):
+        for y in range(ly):
+            for x in range(lx):
+                nx, ny, nz = float(x), x + y + math.cos(z), 0.0
+                a = math.sqrt(nx * nx + ny * ny + nz * nz)
+                if a == 0:
+                    nx, ny, nz = 1.0, 0.0, 0.0
+                else:
+                    nx, ny, nz = nx / a, ny / a, nz / a
+                out[i] = (nx, ny, nz, 0.53 + 0.0000000001 * (x + y + z))
+                i += 1
+    return out

@@ -35,7 +35,7 @@ def summary(p):
 
 def test_driver_rejects_unknown_and_unsupported_flags():
     assert os.path.exists(DRIVER), "build the package first"
-    for args in (["--bogus", "3"], ["--linearSpacedSaving", "10"], ["-z", "2"], ["--stride", "2"],
+    for args in (["--bogus", "3"], ["--linearSpacedSaving", "10"], ["-z", "4"], ["-z", "-1"], ["--stride", "2"],
                  ["--wall", "2,0"], ["--precision", "fp16"]):
         p = run(*args, check=False)
         assert p.returncode == 1
@@ -116,3 +116,32 @@ def test_driver_mixed_precision_polish_reaches_fp64_answer():
     itm, fmm, em = summary(run(*common, "--precision", "mixed"))
     assert fm64 <= 1e-11 and fmm <= 1e-11
     assert em == pytest.approx(e64, rel=1e-9)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("z", [1, 2, 3])
+def test_driver_initial_condition_modes(z):
+    """-z 1 (one random director from the seeded stream), -z 2 (the fixed
+    director (1/2, 0, 1/2)), -z 3 (the example director function of
+    setInitialConditions.h), then a homeotropic sphere (a uniform texture alone
+    is already stationary) and 20 FIRE steps, against the oracle started the
+    same way."""
+    from conftest import example_directors
+    import math
+    p = run("-l", 16, "-i", 20, "--L1", 2.32, "--randomSeed", 99, "-z", z, "--sphere", "8,8,8,4,0,5.3")
+    it, fm, e = summary(p)
+    o = po.OracleSystem(16, 16, 16)
+    if z == 1:
+        o.random_uniform_init(99, S0)
+    elif z == 2:
+        a = math.sqrt(0.5 * 0.5 + 0.0 + 0.5 * 0.5)
+        o.director_init(np.tile([0.5 / a, 0.0, 0.5 / a, S0], (o.n, 1)))
+    else:
+        o.director_init(example_directors((16, 16, 16)))
+    o.sphere((8.0, 8.0, 8.0), 4.0, 0, 5.3, S0)
+    o.set_params(A, B, C, 2.32, 0, 0, semantics=1)
+    it_ref, tr = o.fire(max_iter=20)
+    e_ref, _ = o.energy()
+    assert it == it_ref == 20
+    assert fm == pytest.approx(tr[-1, 0], rel=1e-10)
+    assert e == pytest.approx(e_ref / o.n_active(), rel=1e-12)

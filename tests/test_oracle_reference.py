@@ -129,6 +129,67 @@ def test_lattice_init_types_defects(golden, phase):
     np.testing.assert_allclose(d[lc], golden["small_defects"][lc], rtol=1e-13, atol=1e-15)
 
 
+@live_ref
+@pytest.mark.parametrize("seed", [1234, 7])
+def test_random_uniform_init_matches_reference(phase, seed):
+    """openQmin -z 1: setNematicQTensorRandomly(noise, S0, globallyAligned=true)
+    of the compiled reference, bit for bit."""
+    A, B, C, S0 = phase
+    r = po.RefModel(6, 5, 4)
+    r.random_init_aligned(seed, S0)
+    o = po.OracleSystem(6, 5, 4)
+    o.random_uniform_init(seed, S0)
+    assert np.array_equal(o.q(), r.q())
+
+
+def test_random_uniform_director_is_the_streams_first_two_draws(phase):
+    """The aligned texture is uniaxial with order S0 along the director of the
+    first two doubles of the seeded stream (theta = acos(2u - 1), phi = 2 pi u):
+    checked through the eigen-decomposition of Q, not the Q formula."""
+    import math
+    A, B, C, S0 = phase
+    raw = np.random.RandomState(99)._bit_generator.random_raw(4).astype(np.float64)
+    u = (raw[0::2] + raw[1::2] * 2.0 ** 32) / 2.0 ** 64
+    theta, phi = math.acos(2 * u[0] - 1), 2 * math.pi * u[1]
+    n = np.array([math.cos(phi) * math.sin(theta), math.sin(phi) * math.sin(theta), math.cos(theta)])
+    o = po.OracleSystem(4, 3, 2)
+    o.random_uniform_init(99, S0)
+    q = o.q()
+    assert np.array_equal(q, np.broadcast_to(q[0], q.shape))
+    Q = np.array([[q[0, 0], q[0, 1], q[0, 2]], [q[0, 1], q[0, 3], q[0, 4]], [q[0, 2], q[0, 4], -q[0, 0] - q[0, 3]]])
+    w, v = np.linalg.eigh(Q)
+    assert abs(abs(v[:, 2] @ n) - 1) < 1e-14
+    np.testing.assert_allclose(w, [-S0 / 2, -S0 / 2, S0], atol=1e-14)
+
+
+@live_ref
+def test_director_function_init_matches_reference(phase):
+    """openQmin -z 3 (setDirectorFromFunction with the example function of
+    setInitialConditions.h): the test helper's normalisation equals the
+    reference's normalizeDirector, and the oracle's Q equals the reference's
+    qTensorFromDirector at every site, bit for bit."""
+    from conftest import example_directors
+    shape = (6, 5, 4)
+    n4 = example_directors(shape)
+    lib = po.ref_lib()
+    i = 0
+    for z in range(shape[2]):
+        for y in range(shape[1]):
+            for x in range(shape[0]):
+                import math
+                v = np.array([float(x), x + y + math.cos(z), 0.0])
+                lib.qmref_normalize_director(po.dptr(v))
+                assert np.array_equal(v, n4[i, :3])
+                i += 1
+    o = po.OracleSystem(*shape)
+    o.director_init(n4)
+    ref = np.zeros((len(n4), 5))
+    for i in range(len(n4)):
+        d = np.ascontiguousarray(n4[i, :3])
+        lib.qmref_q_from_director(po.dptr(d), n4[i, 3], po.dptr(ref[i]))
+    assert np.array_equal(o.q(), ref)
+
+
 def test_uniform_stream_is_libstdcxx_generate_canonical(golden):
     """getRealUniform = uniform_real_distribution<double> over mt19937: two 32-bit
     draws per double, (a + b 2^32) / 2^64 (libstdc++ generate_canonical).  numpy's
