@@ -122,6 +122,10 @@ int qm_lattice_defects(qm_lattice* lat, int kind, double* out);
  * (src/simulation/multirankSimulation.cpp:573-665): lines "x y z Qxx Qxy Qxz Qyy Qyz type Smax";
  * path is the full file name */
 int qm_lattice_save_state(qm_lattice* lat, const char* path);
+/* multirankSimulation::saveState(fname, latticeSkip, defectType)
+ * (multirankSimulation.cpp:630-665): only sites whose x, y and z are multiples
+ * of latticeSkip; the last column is computeDefectMeasures(defectType) */
+int qm_lattice_save_state_ex(qm_lattice* lat, const char* path, int latticeSkip, int defectType);
 int qm_lattice_load_state(qm_lattice* lat, const char* path);
 /* copy the device state Q of one lattice into another of the same size */
 int qm_lattice_copy_q(qm_lattice* dst, const qm_lattice* src);

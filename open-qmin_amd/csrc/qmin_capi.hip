@@ -569,23 +569,31 @@ int qm_lattice_defects(qm_lattice* L, int kind, double* out) {
     return QM_OK;
 }
 
-// saveState (multirankSimulation.cpp:630-665): "x y z Q0 Q1 Q2 Q3 Q4 type Smax",
-// tab separated, one line per site in index order
-int qm_lattice_save_state(qm_lattice* L, const char* path) {
+// saveState (multirankSimulation.cpp:630-665): "x y z Q0 Q1 Q2 Q3 Q4 type defect",
+// tab separated, one line per saved site in index order, numbers as an
+// ostream writes them (6 significant digits, %g); sites with x, y and z all
+// multiples of latticeSkip; defect = computeDefectMeasures(defectType) on
+// liquid-crystal sites, 0 on object sites (never computed there)
+int qm_lattice_save_state_ex(qm_lattice* L, const char* path, int latticeSkip, int defectType) {
     if (!L || !path) return fail(QM_EINVAL, "bad argument");
-    std::vector<double> q(5 * L->n), smax(L->n);
+    if (defectType < 0 || defectType > 2) return fail(QM_EINVAL, "defect type is 0, 1 or 2");
+    const int stride = latticeSkip < 1 ? 1 : latticeSkip;
+    std::vector<double> q(5 * L->n), defects(L->n);
     if (int rc = qm_lattice_get_field(L, QM_FIELD_Q, q.data())) return rc;
-    if (int rc = qm_lattice_defects(L, 0, smax.data())) return rc;
+    if (int rc = qm_lattice_defects(L, defectType, defects.data())) return rc;
     FILE* f = std::fopen(path, "w");
     if (!f) return fail(QM_EIO, std::string("cannot write ") + path);
     for (long long i = 0; i < L->n; ++i) {
         const int x = int(i % L->lx), y = int((i / L->lx) % L->ly), z = int(i / ((long long)L->lx * L->ly));
-        std::fprintf(f, "%i\t%i\t%i\t%f\t%f\t%f\t%f\t%f\t%i\t%f\n", x, y, z, q[5 * i], q[5 * i + 1], q[5 * i + 2],
-                     q[5 * i + 3], q[5 * i + 4], L->types[i], smax[i]);
+        if (x % stride != 0 || y % stride != 0 || z % stride != 0) continue;
+        std::fprintf(f, "%i\t%i\t%i\t%g\t%g\t%g\t%g\t%g\t%i\t%g\n", x, y, z, q[5 * i], q[5 * i + 1], q[5 * i + 2],
+                     q[5 * i + 3], q[5 * i + 4], L->types[i], L->types[i] > 0 ? 0.0 : defects[i]);
     }
-    std::fclose(f);
+    if (std::fclose(f) != 0) return fail(QM_EIO, std::string("cannot write ") + path);
     return QM_OK;
 }
+
+int qm_lattice_save_state(qm_lattice* L, const char* path) { return qm_lattice_save_state_ex(L, path, 1, 0); }
 
 // loadState (multirankSimulation.cpp:573-612): reads Q only; types are ignored
 int qm_lattice_load_state(qm_lattice* L, const char* path) {

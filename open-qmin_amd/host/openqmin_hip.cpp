@@ -4,7 +4,8 @@
 // Flags keep the reference's names and defaults (openQmin.cpp:59-115):
 //   -l/--boxL, --Lx/--Ly/--Lz, -i/--iterations, -f/--fTarget, -e/--deltaT,
 //   -a/-b/-c phase constants, --L1 --L2 --L3 --L4 --L6, --randomSeed,
-//   --boundaryFile, --saveFile, --initialConfigurationFile, --spatiallyVaryingFieldFile,
+//   --boundaryFile, --saveFile, --linearSpacedSaving, --logSpacedSaving, --stride,
+//   --initialConfigurationFile, --spatiallyVaryingFieldFile,
 //   --hField{X,Y,Z,Mu0,Chi,DeltaChi},
 //   --eField{X,Y,Z,Epsilon0,Epsilon,DeltaEpsilon}, -z/--initializationSwitch (-1, 0, 1, 2, 3 as
 //   setInitialConditions.h:21-99, with its example director function for 3), -v.
@@ -77,9 +78,6 @@ Args parse(int argc, char** argv) {
         bool known = false;
         for (const char* name : kValued) known |= key == name;
         if (!known) throw std::runtime_error("unknown flag --" + key);
-        if ((key == "linearSpacedSaving" || key == "logSpacedSaving") && std::atof(val.c_str()) > 0)
-            throw std::runtime_error("save schedules are not provided: minimize and save the end state");
-        if (key == "stride" && std::atoi(val.c_str()) != 1) throw std::runtime_error("only --stride 1 is provided");
         if (key == "minimizer" && val != "fire" && val != "adam" && val != "nesterov" && val != "lolbfgs")
             throw std::runtime_error("unknown --minimizer " + val);
         if (key == "wall") {
@@ -100,6 +98,8 @@ Args parse(int argc, char** argv) {
     if (a.has("precision") && a.v.at("precision") != "fp64" && a.v.at("precision") != "fp32" &&
         a.v.at("precision") != "mixed")
         throw std::runtime_error("--precision is fp64, fp32 or mixed");
+    if (a.s("precision", "fp64") == "mixed" && (a.i("linearSpacedSaving", -1) > 0 || a.d("logSpacedSaving", -1) > 0))
+        throw std::runtime_error("--precision mixed does not combine with save schedules");
     return a;
 }
 
@@ -224,7 +224,37 @@ int main(int argc, char** argv) {
         sim->finalizeObjects();
 
         const auto t0 = std::chrono::steady_clock::now();
-        sim->performTimestep();
+        // openQmin.cpp:291-333: minimize, or minimize in stages saving the state
+        // before each (file names carry the iteration, "_t<iteration>")
+        const int linearSave = a.i("linearSpacedSaving", -1);
+        const scalar logSave = a.d("logSpacedSaving", -1);
+        const int saveStride = a.i("stride", 1);
+        const std::string saveFile = a.s("saveFile", "NONE");
+        if (linearSave < 0 && logSave < 0) {
+            sim->performTimestep();
+        } else if (logSave < 0) {
+            int currentIteration = 0;
+            while (currentIteration + linearSave < maximumIterations && upd->getMaxForce() > forceCutoff) {
+                const std::string newSaveFile = saveFile + "_t" + std::to_string(currentIteration);
+                if (saveFile != "NONE") sim->saveState(newSaveFile, saveStride);
+                currentIteration += linearSave;
+                upd->setMaximumIterations(currentIteration);
+                sim->performTimestep();
+                if (verbose) std::printf("saving to %s (plus _xX_yY_zZ.txt)\n", newSaveFile.c_str());
+            }
+        } else {
+            int currentIteration = 0;
+            logSpacedIntegers lsi(0, logSave);
+            while (currentIteration < maximumIterations && upd->getMaxForce() > forceCutoff) {
+                const std::string newSaveFile = saveFile + "_t" + std::to_string(currentIteration);
+                if (saveFile != "NONE") sim->saveState(newSaveFile, saveStride);
+                lsi.update();
+                currentIteration = lsi.nextSave;
+                upd->setMaximumIterations(currentIteration);
+                sim->performTimestep();
+                if (verbose) std::printf("saving to %s (plus _xX_yY_zZ.txt)\n", newSaveFile.c_str());
+            }
+        }
         if (precision == "mixed") {
             // FP64 polish: the state converted in place, the minimizer continues
             // from where the FP32 phase stopped
@@ -236,7 +266,7 @@ int main(int argc, char** argv) {
         const scalar e = sim->computePotentialEnergy(verbose);
         std::printf("iterations %d forceMax %.17g energy %.17g seconds %.6f\n", upd->iterations, upd->getMaxForce(),
                     e, seconds);
-        if (a.s("saveFile", "NONE") != "NONE") sim->saveState(a.s("saveFile", ""));
+        if (saveFile != "NONE") sim->saveState(saveFile, saveStride);
         return 0;
     } catch (const std::exception& ex) {
         std::fprintf(stderr, "openQmin-hip: %s\n", ex.what());

@@ -97,6 +97,38 @@ class noiseSource {
     int seed_ = 13377;
 };
 
+// logSpacedIntegers (inc/logSpacedIntegers.h): the iterations of logarithmically
+// spaced saves, round(10^(exponent k)) for k = 1, 2, ..., repeats skipped
+class logSpacedIntegers {
+   public:
+    logSpacedIntegers(int firstSave = 0, scalar _exp = 0.05) {
+        nextSave = firstSave;
+        exponent = _exp;
+        base = std::pow(10.0, exponent);
+        logSaveIdx = 0;
+        if (nextSave != 0) {
+            int tempCur = 0;
+            while (tempCur < nextSave) {
+                logSaveIdx += 1;
+                tempCur = (int)std::round(std::pow(base, logSaveIdx));
+            }
+        }
+    }
+    void update() {
+        logSaveIdx += 1;
+        int curSave = (int)std::round(std::pow(base, logSaveIdx));
+        while (curSave == nextSave) {
+            logSaveIdx += 1;
+            curSave = (int)std::round(std::pow(base, logSaveIdx));
+        }
+        nextSave = curSave;
+    }
+    int nextSave;
+    int logSaveIdx;
+    scalar exponent;
+    scalar base;
+};
+
 // ---------------------------------------------------------------- models
 class simpleModel {
    public:
@@ -542,10 +574,8 @@ class multirankSimulation {
 
     // multirankSimulation.cpp:573-665, rank (0,0,0) file name
     void saveState(std::string fname, int latticeSkip = 1, int defectType = 0) {
-        if (latticeSkip != 1 || defectType != 0)
-            throw qminError(QM_EINVAL, "saveState: only latticeSkip 1 and defect type 0 are provided");
         const std::string fn = fname + "_x0y0z0.txt";
-        qmin_host::check(qm_lattice_save_state(lat(), fn.c_str()), "saveState");
+        qmin_host::check(qm_lattice_save_state_ex(lat(), fn.c_str(), latticeSkip, defectType), "saveState");
     }
     void loadState(std::string fname) {
         const std::string fn = fname + "_x0y0z0.txt";

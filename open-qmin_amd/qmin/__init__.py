@@ -80,6 +80,7 @@ _API = {
     "qm_lattice_n_boundary": [_VP, _L],
     "qm_lattice_defects": [_VP, _i, _D],
     "qm_lattice_save_state": [_VP, ctypes.c_char_p],
+    "qm_lattice_save_state_ex": [_VP, ctypes.c_char_p, _i, _i],
     "qm_lattice_load_state": [_VP, ctypes.c_char_p],
     "qm_lattice_copy_q": [_VP, _VP],
     "qm_ldg_create": [ctypes.POINTER(_VP)],
@@ -310,8 +311,10 @@ class Lattice:
         _check(self.lib.qm_lattice_defects(self.h, kind, _dp(out)))
         return out
 
-    def save_state(self, path):
-        _check(self.lib.qm_lattice_save_state(self.h, path.encode()))
+    def save_state(self, path, stride=1, defect_type=0):
+        """saveState's text format (one line per site whose coordinates are all
+        multiples of stride; last column computeDefectMeasures(defect_type))."""
+        _check(self.lib.qm_lattice_save_state_ex(self.h, path.encode(), stride, defect_type))
 
     def load_state(self, path):
         _check(self.lib.qm_lattice_load_state(self.h, path.encode()))

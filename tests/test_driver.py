@@ -35,7 +35,7 @@ def summary(p):
 
 def test_driver_rejects_unknown_and_unsupported_flags():
     assert os.path.exists(DRIVER), "build the package first"
-    for args in (["--bogus", "3"], ["--linearSpacedSaving", "10"], ["-z", "4"], ["-z", "-1"], ["--stride", "2"],
+    for args in (["--bogus", "3"], ["-z", "4"], ["-z", "-1"], ["--precision", "mixed", "--linearSpacedSaving", "5"],
                  ["--wall", "2,0"], ["--precision", "fp16"]):
         p = run(*args, check=False)
         assert p.returncode == 1
@@ -145,3 +145,30 @@ def test_driver_initial_condition_modes(z):
     assert it == it_ref == 20
     assert fm == pytest.approx(tr[-1, 0], rel=1e-10)
     assert e == pytest.approx(e_ref / o.n_active(), rel=1e-12)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("schedule", [("--linearSpacedSaving", "7"), ("--logSpacedSaving", "0.3")])
+def test_driver_save_schedules(tmp_path, schedule):
+    """openQmin.cpp:291-333: staged minimization with a state saved before each
+    stage, named <saveFile>_t<iteration>; the stages follow the reference's loop
+    (linear: while it + k < max; log: round(10^(0.3 j)) = 2, 4, 8, 16, 32) and
+    FIRE continues across them as one run; --stride thins the saved sites."""
+    base = str(tmp_path / "st")
+    p = run("-l", 12, "-i", 30, "--L1", 2.32, "--randomSeed", 5, "--saveFile", base, "--stride", 2, *schedule)
+    it, fm, e = summary(p)
+    saved = sorted(int(f.name.split("_t")[1].split("_")[0]) for f in tmp_path.glob("st_t*_x0y0z0.txt"))
+    if schedule[0] == "--linearSpacedSaving":
+        assert saved == [0, 7, 14, 21] and it == 28
+    else:
+        assert saved == [0, 2, 4, 8, 16] and it == 32
+    o = po.OracleSystem(12, 12, 12)
+    o.random_init(5, S0)
+    o.set_params(A, B, C, 2.32, 0, 0, semantics=1)
+    it_ref, tr = o.fire(max_iter=it)
+    assert fm == pytest.approx(tr[-1, 0], rel=1e-10)
+    rows = np.loadtxt(str(tmp_path / "st_x0y0z0.txt"))
+    assert rows.shape == (6 ** 3, 10)
+    assert np.all(rows[:, :3] % 2 == 0)
+    q = o.q().reshape(12, 12, 12, 5)[::2, ::2, ::2].reshape(-1, 5)
+    np.testing.assert_allclose(rows[:, 3:8], q, rtol=1e-5, atol=1e-6)
