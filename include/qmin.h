@@ -277,7 +277,9 @@ int qm_ws_conv_forward(const void* x, int cin, const void* wt, const float* bias
  * ceil(n/chunk) * 27 * cout * cin floats of per-chunk sums (chunk a multiple of 16) */
 int qm_ws_conv_wgrad(const void* g, int cout, const void* x, int cin, int reluIn, int lx, int ly, int lz, int dil,
                      float* part, long long chunk, float* dw, void* stream);
-int qm_ws_bias_grad(const void* g, long long n, int cout, float* db, void* stream);
+/* db[co] = sum_s g[s][co]; part holds blocks * cout floats of per-block sums,
+ * added in a fixed order */
+int qm_ws_bias_grad(const void* g, long long n, int cout, float* part, int blocks, float* db, void* stream);
 /* Adam step `step` (1-based) on FP32 weights; refreshes the BF16 copy and, when
  * wbd16 is not NULL, the mirrored transpose wbd[26-tap][ci][co] used for input gradients */
 int qm_ws_adam(float* w, const float* g, float* m, float* v, long long count, float lr, float b1, float b2, float eps,

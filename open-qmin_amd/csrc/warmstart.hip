@@ -6,9 +6,10 @@
 // Kernels (BF16 operands on the matrix cores, FP32 accumulation):
 //  * conv_fwd_kernel: 3x3x3 dilated periodic convolution as an implicit GEMM,
 //    D[site][cout] = sum_{tap, cin} X[site + dil*tap][cin] Wt[tap][cout][cin],
-//    with v_mfma_f32_16x16x16_bf16: a wave owns 16 consecutive sites and all
+//    with v_mfma_f32_16x16x16_bf16: a wave owns 64 consecutive sites and all
 //    output channels; each lane feeds 4 consecutive input channels of one
-//    neighbour site (activations are channels-last, so that is one 8-byte load).
+//    neighbour site (activations are channels-last, so that is one 8-byte load)
+//    and the tap's weights come from LDS.
 //    Optional ReLU on the input as it is read, bias, and an output mask (the
 //    ReLU derivative of the previous layer when the kernel runs backwards).
 //  * The input gradient is the same kernel with the taps mirrored and the
@@ -61,64 +62,107 @@ struct ConvArgs {
     int cin, cout, lx, ly, lz, dil, reluIn;
 };
 
+constexpr int kFwdMT = 4;   // 16-site row tiles per wave in the forward kernel
+
+// Each wave owns 64 consecutive sites (kFwdMT tiles of 16) and all output
+// channels: per tap and 16-channel slice it gathers one 8-byte A fragment per
+// tile and reads each B fragment once from LDS for all four tiles.  The
+// work-group stages a tap's weights [cout][cin] in LDS (rows padded by 4 BF16
+// against bank conflicts), double-buffered across taps.
 template <int NT, int KT>
 __global__ void __launch_bounds__(kWsThreads) conv_fwd_kernel(ConvArgs a) {
+    constexpr int CO = NT * 16, CI = KT * 16, RS = CI + 4;   // LDS row stride in BF16
+    __shared__ uint2 sW[2][CO * RS / 4];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const long long sbase = ((long long)blockIdx.x * (kWsThreads / 64) + wave) * 16;
-    if (sbase >= a.n) return;
-    const long long s = sbase + (lane & 15);
-    const bool valid = s < a.n;
-    const long long sc = valid ? s : sbase;
-    const int x = int(sc % a.lx), y = int((sc / a.lx) % a.ly), z = int(sc / ((long long)a.lx * a.ly));
-    const int kq = lane >> 4;
-    v4f acc[NT];
+    const int kq = lane >> 4, l16 = lane & 15;
+    const long long wbase = ((long long)blockIdx.x * (kWsThreads / 64) + wave) * (16 * kFwdMT);
+    const bool waveActive = wbase < a.n;
+    int xs[kFwdMT], ys[kFwdMT], zs[kFwdMT];
+    bool valid[kFwdMT];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (int mt = 0; mt < kFwdMT; ++mt) {
+        const long long s = wbase + 16 * mt + l16;
+        valid[mt] = s < a.n;
+        const long long sc = valid[mt] ? s : 0;
+        xs[mt] = int(sc % a.lx);
+        ys[mt] = int((sc / a.lx) % a.ly);
+        zs[mt] = int(sc / ((long long)a.lx * a.ly));
+    }
+    v4f acc[kFwdMT][NT];
+#pragma unroll
+    for (int mt = 0; mt < kFwdMT; ++mt)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[mt][t] = v4f{0.f, 0.f, 0.f, 0.f};
+
+    auto stage = [&](int tap, int buf) {
+        const uint2* src = reinterpret_cast<const uint2*>(a.wt + (long long)tap * CO * CI);
+        for (int e = threadIdx.x; e < CO * CI / 4; e += kWsThreads) {
+            const int co = e / (CI / 4), c4 = e % (CI / 4);
+            sW[buf][co * (RS / 4) + c4] = src[e];
+        }
+    };
+    stage(0, 0);
+    __syncthreads();
     for (int tap = 0; tap < kTaps; ++tap) {
-        const long long nb = tap_site(x, y, z, tap, a.dil, a.lx, a.ly, a.lz);
-        const uint16_t* xr = a.x + nb * (KT * 16) + kq * 4;
-        const uint16_t* wr = a.wt + (long long)tap * (NT * 16) * (KT * 16) + (long long)(lane & 15) * (KT * 16) + kq * 4;
+        const int buf = tap & 1;
+        // the other buffer was last read in tap - 1, before the barrier below
+        if (tap + 1 < kTaps) stage(tap + 1, buf ^ 1);
+        if (waveActive) {
+            const uint16_t* xr[kFwdMT];
 #pragma unroll
-        for (int kt = 0; kt < KT; ++kt) {
-            const uint2 raw = *reinterpret_cast<const uint2*>(xr + kt * 16);
-            v4s av;
-            av[0] = short(raw.x & 0xffffu);
-            av[1] = short(raw.x >> 16);
-            av[2] = short(raw.y & 0xffffu);
-            av[3] = short(raw.y >> 16);
-            if (a.reluIn) {
+            for (int mt = 0; mt < kFwdMT; ++mt)
+                xr[mt] = a.x + tap_site(xs[mt], ys[mt], zs[mt], tap, a.dil, a.lx, a.ly, a.lz) * CI + kq * 4;
 #pragma unroll
-                for (int j = 0; j < 4; ++j) av[j] = av[j] < 0 ? short(0) : av[j];
-            }
-            if (!valid) av = v4s{0, 0, 0, 0};
+            for (int kt = 0; kt < KT; ++kt) {
+                v4s av[kFwdMT];
 #pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                const uint2 wraw = *reinterpret_cast<const uint2*>(wr + (long long)t * 16 * (KT * 16) + kt * 16);
-                v4s bv;
-                bv[0] = short(wraw.x & 0xffffu);
-                bv[1] = short(wraw.x >> 16);
-                bv[2] = short(wraw.y & 0xffffu);
-                bv[3] = short(wraw.y >> 16);
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[t], 0, 0, 0);
+                for (int mt = 0; mt < kFwdMT; ++mt) {
+                    const uint2 raw = *reinterpret_cast<const uint2*>(xr[mt] + kt * 16);
+                    av[mt][0] = short(raw.x & 0xffffu);
+                    av[mt][1] = short(raw.x >> 16);
+                    av[mt][2] = short(raw.y & 0xffffu);
+                    av[mt][3] = short(raw.y >> 16);
+                    if (a.reluIn) {
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) av[mt][j] = av[mt][j] < 0 ? short(0) : av[mt][j];
+                    }
+                    if (!valid[mt]) av[mt] = v4s{0, 0, 0, 0};
+                }
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    const uint2 wraw = sW[buf][((t * 16 + l16) * RS + kt * 16 + kq * 4) / 4];
+                    v4s bv;
+                    bv[0] = short(wraw.x & 0xffffu);
+                    bv[1] = short(wraw.x >> 16);
+                    bv[2] = short(wraw.y & 0xffffu);
+                    bv[3] = short(wraw.y >> 16);
+#pragma unroll
+                    for (int mt = 0; mt < kFwdMT; ++mt)
+                        acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[mt], bv, acc[mt][t], 0, 0, 0);
+                }
             }
         }
+        __syncthreads();
     }
-    const int co0 = lane & 15;
+    if (!waveActive) return;
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-        const int co = t * 16 + co0;
+        const int co = t * 16 + l16;
         const float b = a.bias ? a.bias[co] : 0.f;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const long long site = sbase + 4 * kq + j;
-            if (site >= a.n) continue;
-            float v = acc[t][j] + b;
-            const long long o = site * (NT * 16) + co;
-            if (a.mask && !(bf2f(a.mask[o]) > 0.f)) v = 0.f;
-            if (a.yf)
-                a.yf[o] = v;
-            else
-                a.ybf[o] = f2bf(v);
+        for (int mt = 0; mt < kFwdMT; ++mt) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const long long site = wbase + 16 * mt + 4 * kq + j;
+                if (site >= a.n) continue;
+                float v = acc[mt][t][j] + b;
+                const long long o = site * CO + co;
+                if (a.mask && !(bf2f(a.mask[o]) > 0.f)) v = 0.f;
+                if (a.yf)
+                    a.yf[o] = v;
+                else
+                    a.ybf[o] = f2bf(v);
+            }
         }
     }
 }
@@ -131,40 +175,92 @@ struct WgradArgs {
     int cin, cout, lx, ly, lz, dil, reluIn;
 };
 
+constexpr int kWgSites = 64;             // sites staged per step of the weight gradient
+constexpr int kWgLd = kWgSites + 8;      // LDS row stride (BF16) of the transposed tiles
+
+// One work-group per (chunk of sites, tap).  Per step it stages 64 sites of the
+// output gradient G and of the tap's neighbours of X (ReLU applied) in LDS,
+// transposed to [channel][site], so that both MFMA operands read 4 consecutive
+// sites (the contraction axis) as one 8-byte LDS load; the four waves split the
+// 16 x 16 output tiles of dW[tap][cout][cin].
 template <int NT, int KT>
 __global__ void __launch_bounds__(kWsThreads) conv_wgrad_kernel(WgradArgs a) {
+    constexpr int CO = NT * 16, CI = KT * 16, TILES = NT * KT;
+    constexpr int TPW = (TILES + 3) / 4;    // tiles per wave
+    __shared__ uint16_t gT[CO * kWgLd];
+    __shared__ uint16_t xT[CI * kWgLd];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int kq = lane >> 4, l16 = lane & 15;
     const int tap = blockIdx.y;
     const long long c = blockIdx.x;
     const long long s0 = c * a.chunk, s1 = min(a.n, s0 + a.chunk);
-    const int kq = lane >> 4, l16 = lane & 15;
-    for (int p = wave; p < NT * KT; p += kWsThreads / 64) {
-        const int mt = p / KT, kt = p % KT;
-        const int co = mt * 16 + l16, ci = kt * 16 + l16;
-        v4f acc = {0.f, 0.f, 0.f, 0.f};
-        for (long long sb = s0; sb < s1; sb += 16) {
-            v4s av, bv;
+    v4f acc[TPW];
+#pragma unroll
+    for (int k = 0; k < TPW; ++k) acc[k] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (long long sb = s0; sb < s1; sb += kWgSites) {
+        // stage: 8-byte loads (4 channels of one site), 2-byte transposed LDS stores
+        for (int e = threadIdx.x; e < kWgSites * CO / 4; e += kWsThreads) {
+            const int site = e / (CO / 4), c4 = (e % (CO / 4)) * 4;
+            const long long sg = sb + site;
+            uint2 raw = {0u, 0u};
+            if (sg < s1) raw = *reinterpret_cast<const uint2*>(a.g + sg * CO + c4);
+            gT[(c4 + 0) * kWgLd + site] = uint16_t(raw.x & 0xffffu);
+            gT[(c4 + 1) * kWgLd + site] = uint16_t(raw.x >> 16);
+            gT[(c4 + 2) * kWgLd + site] = uint16_t(raw.y & 0xffffu);
+            gT[(c4 + 3) * kWgLd + site] = uint16_t(raw.y >> 16);
+        }
+        for (int e = threadIdx.x; e < kWgSites * CI / 4; e += kWsThreads) {
+            const int site = e / (CI / 4), c4 = (e % (CI / 4)) * 4;
+            const long long sg = sb + site;
+            uint2 raw = {0u, 0u};
+            if (sg < s1) {
+                const int x = int(sg % a.lx), y = int((sg / a.lx) % a.ly), z = int(sg / ((long long)a.lx * a.ly));
+                const long long nb = tap_site(x, y, z, tap, a.dil, a.lx, a.ly, a.lz);
+                raw = *reinterpret_cast<const uint2*>(a.x + nb * CI + c4);
+            }
+            uint16_t v[4] = {uint16_t(raw.x & 0xffffu), uint16_t(raw.x >> 16), uint16_t(raw.y & 0xffffu),
+                             uint16_t(raw.y >> 16)};
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const long long site = sb + 4 * kq + j;
-                short ga = 0, xb = 0;
-                if (site < s1) {
-                    ga = short(a.g[site * (NT * 16) + co]);
-                    const int x = int(site % a.lx), y = int((site / a.lx) % a.ly),
-                              z = int(site / ((long long)a.lx * a.ly));
-                    const long long nb = tap_site(x, y, z, tap, a.dil, a.lx, a.ly, a.lz);
-                    xb = short(a.x[nb * (KT * 16) + ci]);
-                    if (a.reluIn && xb < 0) xb = 0;
-                }
-                av[j] = ga;
-                bv[j] = xb;
+                if (a.reluIn && (v[j] & 0x8000u)) v[j] = 0;   // negative BF16 (and -0) -> +0
+                xT[(c4 + j) * kWgLd + site] = v[j];
             }
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
         }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < TPW; ++k) {
+            const int tile = wave * TPW + k;
+            if (tile >= TILES) break;
+            const int mt = tile / KT, nt = tile % KT;
+#pragma unroll
+            for (int ks = 0; ks < kWgSites / 16; ++ks) {
+                const int site = ks * 16 + 4 * kq;
+                const uint2 ar = *reinterpret_cast<const uint2*>(&gT[(mt * 16 + l16) * kWgLd + site]);
+                const uint2 br = *reinterpret_cast<const uint2*>(&xT[(nt * 16 + l16) * kWgLd + site]);
+                v4s av, bv;
+                av[0] = short(ar.x & 0xffffu);
+                av[1] = short(ar.x >> 16);
+                av[2] = short(ar.y & 0xffffu);
+                av[3] = short(ar.y >> 16);
+                bv[0] = short(br.x & 0xffffu);
+                bv[1] = short(br.x >> 16);
+                bv[2] = short(br.y & 0xffffu);
+                bv[3] = short(br.y >> 16);
+                acc[k] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[k], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    // D[m = cout][n = cin]: lane l, element j = D[4 (l / 16) + j][l % 16]
+#pragma unroll
+    for (int k = 0; k < TPW; ++k) {
+        const int tile = wave * TPW + k;
+        if (tile >= TILES) break;
+        const int mt = tile / KT, nt = tile % KT;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const int row = mt * 16 + 4 * kq + j;   // cout
-            a.part[((c * kTaps + tap) * (NT * 16) + row) * (KT * 16) + ci] = acc[j];
+            const int row = mt * 16 + 4 * kq + j;
+            a.part[((c * kTaps + tap) * CO + row) * CI + nt * 16 + l16] = acc[k][j];
         }
     }
 }
@@ -178,19 +274,24 @@ __global__ void wgrad_reduce_kernel(const float* part, long long chunks, long lo
     }
 }
 
-// bias gradient: one work-group per output channel, fixed-order tree
-__global__ void __launch_bounds__(kWsThreads) bias_grad_kernel(const uint16_t* g, long long n, int cout, float* db) {
+// bias gradient, first stage: work-group b sums its chunk of sites for every
+// channel (consecutive threads read consecutive channels of a site: coalesced);
+// wgrad_reduce_kernel then adds the work-groups' rows in a fixed order
+__global__ void __launch_bounds__(kWsThreads) bias_grad_kernel(const uint16_t* g, long long n, int cout,
+                                                              long long chunk, float* part) {
     __shared__ float red[kWsThreads];
-    const int co = blockIdx.x;
+    const int rows = kWsThreads / cout, c = threadIdx.x % cout, r = threadIdx.x / cout;
+    const long long s0 = (long long)blockIdx.x * chunk, s1 = min(n, s0 + chunk);
     float s = 0.f;
-    for (long long i = threadIdx.x; i < n; i += kWsThreads) s += bf2f(g[i * cout + co]);
+    if (r < rows)
+        for (long long i = s0 + r; i < s1; i += rows) s += bf2f(g[i * cout + c]);
     red[threadIdx.x] = s;
     __syncthreads();
-    for (int w = kWsThreads / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
+    if (r == 0) {
+        float t = 0.f;
+        for (int k = 0; k < rows; ++k) t += red[k * cout + c];
+        part[(long long)blockIdx.x * cout + c] = t;
     }
-    if (threadIdx.x == 0) db[co] = red[0];
 }
 
 // Adam on FP32 master weights; refreshes the BF16 copy and, for [27][cout][cin]
@@ -333,7 +434,7 @@ inline int grid_for(long long work, int per = kWsThreads) {
 
 template <int NT, int KT>
 hipError_t launch_fwd_t(const ConvArgs& a, hipStream_t s) {
-    const long long waves = (a.n + 15) / 16;
+    const long long waves = (a.n + 16 * kFwdMT - 1) / (16 * kFwdMT);
     const int blocks = int((waves + 3) / 4);
     hipLaunchKernelGGL((conv_fwd_kernel<NT, KT>), dim3(blocks), dim3(kWsThreads), 0, s, a);
     return hipGetLastError();
@@ -426,10 +527,14 @@ int qm_ws_conv_wgrad(const void* g, int cout, const void* x, int cin, int reluIn
     return QM_OK;
 }
 
-int qm_ws_bias_grad(const void* g, long long n, int cout, float* db, void* stream) {
-    if (!g || !db || cout < 1) return fail(QM_EINVAL, "bad argument");
-    hipLaunchKernelGGL(bias_grad_kernel, dim3(cout), dim3(kWsThreads), 0, static_cast<hipStream_t>(stream),
-                       static_cast<const uint16_t*>(g), n, cout, db);
+int qm_ws_bias_grad(const void* g, long long n, int cout, float* part, int blocks, float* db, void* stream) {
+    if (!g || !db || !part || !channels_ok(cout) || blocks < 1 || n < 1) return fail(QM_EINVAL, "bad argument");
+    const long long chunk = (n + blocks - 1) / blocks;
+    const auto st = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(bias_grad_kernel, dim3(blocks), dim3(kWsThreads), 0, st, static_cast<const uint16_t*>(g), n, cout,
+                       chunk, part);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(wgrad_reduce_kernel, dim3(1), dim3(64), 0, st, part, (long long)blocks, (long long)cout, db);
     HIP_TRY(hipGetLastError());
     return QM_OK;
 }

@@ -49,7 +49,7 @@ _API = {
     "qm_ldg_set_varying_hfield": [_VP, _i, _d, _d, _d],
     "qm_ws_conv_forward": [_VP, _i, _VP, _VP, _i, _VP, _VP, _VP, _i, _i, _i, _i, _i, _VP],
     "qm_ws_conv_wgrad": [_VP, _i, _VP, _i, _i, _i, _i, _i, _i, _VP, ctypes.c_longlong, _VP, _VP],
-    "qm_ws_bias_grad": [_VP, ctypes.c_longlong, _i, _VP, _VP],
+    "qm_ws_bias_grad": [_VP, ctypes.c_longlong, _i, _VP, _i, _VP, _VP],
     "qm_ws_adam": [_VP, _VP, _VP, _VP, ctypes.c_longlong, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                    ctypes.c_float, _i, _VP, _VP, _i, _i, _VP],
     "qm_ws_weights_to_bf16": [_VP, ctypes.c_longlong, _VP, _VP, _i, _i, _VP],

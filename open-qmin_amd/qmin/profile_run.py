@@ -9,6 +9,11 @@
       the full-elasticity workload (qmin.workloads.full_elasticity: L1..L4 and
       L6, a uniform E field, four homeotropic spheres); times per iteration of
       FIRE, Adam, Nesterov and online L-BFGS (memory 8).
+  python open-qmin_amd/qmin/profile_run.py warmstart --size 128 [--steps 5]
+      the warm-start network's BF16 matrix-core convolutions on an L^3 lattice
+      with five spheres: --steps forward passes (inference), then --steps
+      weight- and input-gradient passes with a unit output gradient; prints the
+      per-layer FLOP counts (2 x 27 x cin x cout per site).
 """
 import argparse
 import os
@@ -45,9 +50,38 @@ def minimizers(qmin, L, precision, name):
               flush=True)
 
 
+def warmstart(L, steps):
+    import time
+    import numpy as np
+    import torch  # noqa: F401  (before the library: one HIP runtime)
+    import qmin
+    from qmin import warmstart as ws
+    from qmin import workloads
+    A, B, C, S0 = workloads.phase_constants()
+    rng = np.random.default_rng(5)
+    lat = qmin.Lattice(L, L, L)
+    lat.random_init(1, S0)
+    for _ in range(5):
+        lat.sphere(tuple(float(v) for v in rng.uniform(0, L, 3)), float(rng.uniform(L / 16, L / 8)), qmin.HOMEOTROPIC,
+                   5.0, S0)
+    net = ws.WarmStartNet(seed=0)
+    feats = ws.lattice_features(lat, (0.0, 0.0, 1.0))
+    dims = lat.shape
+    for _ in range(steps):
+        acts = net.forward(feats, dims)
+    torch.cuda.synchronize()
+    gout = torch.full((lat.n * 16,), 0x3f80, dtype=torch.int16, device="cuda")   # BF16 1.0
+    for _ in range(steps):
+        net.backward(acts, gout, dims)
+    torch.cuda.synchronize()
+    for li, (cin, cout, dil) in enumerate(ws.LAYERS):
+        print("layer %d: %d -> %d channels, dilation %d: %.3e FLOP per forward pass" % (
+            li, cin, cout, dil, 2.0 * 27 * cin * cout * lat.n), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["fire", "copy", "minimizers"])
+    ap.add_argument("what", choices=["fire", "copy", "minimizers", "warmstart"])
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--steps", type=int, default=10)
@@ -69,6 +103,9 @@ def main():
         ms = t0.elapsed_time(t1) / args.steps
         print("copied %d MiB x %d: %.3f ms per copy, %.1f GB/s read+write"
               % (args.mib, args.steps, ms, 2 * args.mib * (1 << 20) / (ms * 1e-3) / 1e9))
+        return
+    if args.what == "warmstart":
+        warmstart(args.size, args.steps)
         return
     import qmin
     if args.what == "minimizers":

@@ -27,6 +27,7 @@ from . import workloads
 
 LAYERS = ((16, 32, 1), (32, 64, 2), (64, 64, 4), (64, 16, 8))   # (cin, cout, dilation)
 WCHUNK = 4096    # sites per weight-gradient partial sum
+BIAS_BLOCKS = 512   # work-groups (partial sums) of the bias gradient
 
 
 def _torch():
@@ -179,7 +180,8 @@ class WarmStartNet:
             part = torch.empty(chunks * 27 * cout * cin, dtype=torch.float32, device="cuda")
             _check(self.lib.qm_ws_conv_wgrad(_ptr(g), cout, _ptr(acts[li]), cin, int(li > 0), dims[0], dims[1],
                                              dims[2], dil, _ptr(part), WCHUNK, _ptr(self.gw[li]), _stream()))
-            _check(self.lib.qm_ws_bias_grad(_ptr(g), n, cout, _ptr(self.gb[li]), _stream()))
+            bpart = torch.empty(BIAS_BLOCKS * cout, dtype=torch.float32, device="cuda")
+            _check(self.lib.qm_ws_bias_grad(_ptr(g), n, cout, _ptr(bpart), BIAS_BLOCKS, _ptr(self.gb[li]), _stream()))
             if li > 0:
                 gin = torch.empty(n * cin, dtype=torch.int16, device="cuda")
                 _check(self.lib.qm_ws_conv_forward(_ptr(g), cout, _ptr(self.wbd[li]), None, cin, _ptr(acts[li]),

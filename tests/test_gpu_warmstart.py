@@ -91,7 +91,8 @@ def test_conv_gradients_match_oracle(cin, cout, dil):
     check(lib.qm_ws_conv_forward(ptr(gd), cout, ptr(wbd), None, cin, ptr(xd), None, ptr(gin), *dims, dil, 0,
                                  stream()))
     db = torch.empty(cout, dtype=torch.float32, device="cuda")
-    check(lib.qm_ws_bias_grad(ptr(gd), n, cout, ptr(db), stream()))
+    bpart = torch.empty(7 * cout, dtype=torch.float32, device="cuda")   # 7 blocks: a ragged last chunk
+    check(lib.qm_ws_bias_grad(ptr(gd), n, cout, ptr(bpart), 7, ptr(db), stream()))
     torch.cuda.synchronize()
     np.testing.assert_allclose(dw.cpu().numpy().reshape(27, cout, cin), ref_dw, rtol=0,
                                atol=1e-5 * np.abs(ref_dw).max())
