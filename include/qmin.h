@@ -121,6 +121,15 @@ int qm_lattice_defects(qm_lattice* lat, int kind, double* out);
 /* multirankSimulation::saveState / loadState text format, one rank
  * (src/simulation/multirankSimulation.cpp:573-665): lines "x y z Qxx Qxy Qxz Qyy Qyz type Smax";
  * path is the full file name */
+/* cubicLattice::boundarySites / surfaceSites (cubicLattice.h:78-81) of object
+ * `object` (0-based, creation order): its sites in creation order (surface = 0)
+ * or its surface sites in order of discovery (surface = 1); *count is always
+ * set, sites (may be NULL) receives *count indices */
+int qm_lattice_object_sites(const qm_lattice* lat, int object, int surface, int64_t* sites, int64_t* count);
+/* cubicLattice::displaceBoundaryObject (cubicLattice.cpp:329-372): moves object
+ * `object` and its surface sites, with their Q, by `magnitude` sites along
+ * direction 0..5 (x-, x+, y-, y+, z-, z+) */
+int qm_lattice_displace_object(qm_lattice* lat, int object, int direction, int magnitude);
 int qm_lattice_save_state(qm_lattice* lat, const char* path);
 /* multirankSimulation::saveState(fname, latticeSkip, defectType)
  * (multirankSimulation.cpp:630-665): only sites whose x, y and z are multiples
@@ -164,6 +173,12 @@ int qm_ldg_compute_forces(qm_ldg* f, qm_lattice* lat);
 /* landauDeGennesLC::computeEnergy (landauDeGennesLC.cpp:204-353): total energy and
  * components {phase, distortion, anchoring, E field, H field}; the reference's
  * computePotentialEnergy reports total / n_active (multirankSimulation.cpp:475-490) */
+/* landauDeGennesLC::computeObjectForces with computeStressTensors
+ * (landauDeGennesLCOtherForces.cpp:147-251): force[3] on object `object`
+ * (0-based) from the stress tensor over its surface sites; this call's force
+ * (the reference adds it to lattice->boundaryForce).  One-constant only, as in
+ * the reference (QM_EINVAL otherwise). */
+int qm_ldg_object_force(qm_ldg* f, qm_lattice* lat, int object, double* force);
 int qm_ldg_compute_energy(qm_ldg* f, qm_lattice* lat, double* total, double* components5);
 
 /* ---- FIRE: energyMinimizerFIRE (src/updaters/energyMinimizerFIRE.h:20-110) */

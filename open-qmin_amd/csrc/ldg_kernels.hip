@@ -703,8 +703,65 @@ __global__ void materialize_velocity_kernel(StateSet<R> s0, StateSet<R> s1, long
     }
 }
 
-// Energy density per site and its components (landauDeGennesLC.cpp:204-353);
-// per-block partial sums of the five components.
+// Energy density of liquid-crystal site i split in its components [phase,
+// distortion, anchoring, E, H] (landauDeGennesLC.cpp:204-353), and the first
+// derivatives there (central; one-sided towards the liquid crystal on an axis
+// with an object neighbour, the upper neighbour taking precedence).
+template <typename R>
+__device__ __forceinline__ void site_energy(const R* q, const int8_t* types, const DevObject* objects, int lx, int ly,
+                                            int lz, const EnergyConst& ec, long long i, double* parts,
+                                            R (*dv)[5]) {
+    const long long n = (long long)lx * ly * lz;
+    const int t = types[i];
+    const int x = int(i % lx), y = int((i / lx) % ly), z = int(i / ((long long)lx * ly));
+    long long nbi[6];
+    nbi[0] = wrapi(x - 1, lx) + (long long)lx * (y + (long long)ly * z);
+    nbi[1] = wrapi(x + 1, lx) + (long long)lx * (y + (long long)ly * z);
+    nbi[2] = x + (long long)lx * (wrapi(y - 1, ly) + (long long)ly * z);
+    nbi[3] = x + (long long)lx * (wrapi(y + 1, ly) + (long long)ly * z);
+    nbi[4] = x + (long long)lx * (y + (long long)ly * wrapi(z - 1, lz));
+    nbi[5] = x + (long long)lx * (y + (long long)ly * wrapi(z + 1, lz));
+    R qc[5], nq[6][5];
+    gload(q, n, i, qc);
+#pragma unroll
+    for (int d = 0; d < 6; ++d) gload(q, n, nbi[d], nq[d]);
+    const R t2 = trQ2(qc);
+    parts[0] = double(R(ec.a) * t2 + R(ec.b) * trQ3(qc) + R(ec.c) * t2 * t2);
+    parts[2] = 0;
+    parts[3] = ec.eOn ? double(field_energy(ec.E, ec.eps, ec.deps, ec.eps0, qc)) : 0.0;
+    parts[4] = ec.hOn ? double(field_energy(ec.H, ec.chi, ec.dchi, ec.mu0, qc)) : 0.0;
+    if (ec.hvar) {
+        const double h3[3] = {ec.hvar[i], ec.hvar[n + i], ec.hvar[2 * n + i]};
+        parts[4] += double(field_energy(h3, ec.chi, ec.dchi, ec.mu0, qc));
+    }
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax)
+#pragma unroll
+        for (int c = 0; c < 5; ++c) dv[ax][c] = R(0.5) * (nq[2 * ax + 1][c] - nq[2 * ax][c]);
+    if (t < 0) {
+        R anch = 0;
+#pragma unroll
+        for (int d = 0; d < 6; ++d) {
+            const int tn = types[nbi[d]];
+            if (tn <= 0) continue;
+            anch += anchoring_energy(objects[tn - 1], qc, nq[d]);
+            const int ax = d / 2;
+#pragma unroll
+            for (int c = 0; c < 5; ++c)
+                dv[ax][c] = (d & 1) ? (qc[c] - nq[d ^ 1][c]) : (nq[d ^ 1][c] - qc[c]);
+        }
+        parts[2] = double(anch);
+    }
+    R e[5];
+    distortion_densities(qc, dv[0], dv[1], dv[2], e);
+    double dist = 0;
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+        if (ec.L[k] != 0) dist += ec.L[k] * double(e[k]);
+    parts[1] = dist;
+}
+
+// Energy: per-block partial sums of the five components over t <= 0 sites.
 template <typename R>
 __global__ void __launch_bounds__(kReduceThreads) energy_kernel(const R* q, const int8_t* types,
                                                                 const DevObject* objects, int lx, int ly, int lz,
@@ -713,54 +770,15 @@ __global__ void __launch_bounds__(kReduceThreads) energy_kernel(const R* q, cons
     const long long n = (long long)lx * ly * lz;
     double acc[5] = {0, 0, 0, 0, 0};
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-        const int t = types[i];
-        if (t > 0) continue;
-        const int x = int(i % lx), y = int((i / lx) % ly), z = int(i / ((long long)lx * ly));
-        long long nbi[6];
-        nbi[0] = wrapi(x - 1, lx) + (long long)lx * (y + (long long)ly * z);
-        nbi[1] = wrapi(x + 1, lx) + (long long)lx * (y + (long long)ly * z);
-        nbi[2] = x + (long long)lx * (wrapi(y - 1, ly) + (long long)ly * z);
-        nbi[3] = x + (long long)lx * (wrapi(y + 1, ly) + (long long)ly * z);
-        nbi[4] = x + (long long)lx * (y + (long long)ly * wrapi(z - 1, lz));
-        nbi[5] = x + (long long)lx * (y + (long long)ly * wrapi(z + 1, lz));
-        R qc[5], nq[6][5];
-        gload(q, n, i, qc);
-#pragma unroll
-        for (int d = 0; d < 6; ++d) gload(q, n, nbi[d], nq[d]);
-        const R t2 = trQ2(qc);
-        acc[0] += double(R(ec.a) * t2 + R(ec.b) * trQ3(qc) + R(ec.c) * t2 * t2);
-        if (ec.eOn) acc[3] += double(field_energy(ec.E, ec.eps, ec.deps, ec.eps0, qc));
-        if (ec.hOn) acc[4] += double(field_energy(ec.H, ec.chi, ec.dchi, ec.mu0, qc));
-        if (ec.hvar) {
-            const double h3[3] = {ec.hvar[i], ec.hvar[n + i], ec.hvar[2 * n + i]};
-            acc[4] += double(field_energy(h3, ec.chi, ec.dchi, ec.mu0, qc));
-        }
+        if (types[i] > 0) continue;
+        double parts[5];
         R dv[3][5];
-#pragma unroll
-        for (int ax = 0; ax < 3; ++ax)
-#pragma unroll
-            for (int c = 0; c < 5; ++c) dv[ax][c] = R(0.5) * (nq[2 * ax + 1][c] - nq[2 * ax][c]);
-        if (t < 0) {
-            R anch = 0;
-#pragma unroll
-            for (int d = 0; d < 6; ++d) {
-                const int tn = types[nbi[d]];
-                if (tn <= 0) continue;
-                anch += anchoring_energy(objects[tn - 1], qc, nq[d]);
-                const int ax = d / 2;
-#pragma unroll
-                for (int c = 0; c < 5; ++c)
-                    dv[ax][c] = (d & 1) ? (qc[c] - nq[d ^ 1][c]) : (nq[d ^ 1][c] - qc[c]);
-            }
-            acc[2] += double(anch);
-        }
-        R e[5];
-        distortion_densities(qc, dv[0], dv[1], dv[2], e);
-        double dist = 0;
-#pragma unroll
-        for (int k = 0; k < 5; ++k)
-            if (ec.L[k] != 0) dist += ec.L[k] * double(e[k]);
-        acc[1] += dist;
+        site_energy(q, types, objects, lx, ly, lz, ec, i, parts, dv);
+        acc[0] += parts[0];
+        acc[3] += parts[3];
+        acc[4] += parts[4];
+        acc[2] += parts[2];
+        acc[1] += parts[1];
     }
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
 #pragma unroll
@@ -774,6 +792,75 @@ __global__ void __launch_bounds__(kReduceThreads) energy_kernel(const R* q, cons
         for (int i = 0; i < kReduceThreads / 64; ++i) s += sRed[i][threadIdx.x];
         partials[5 * (long long)blockIdx.x + threadIdx.x] = s;
     }
+}
+
+// Force on an object from its surface sites (computeObjectForces with
+// computeStressTensors, landauDeGennesLCOtherForces.cpp:147-251; one-constant):
+// per surface site, the stress tensor of the first derivatives and L1 with the
+// site's energy density on the diagonal, applied to the surface-area vector
+// (+-1 per axis towards an object neighbour, the upper one taking precedence).
+// out[3 k + 0..2] is site k's contribution; the host adds them in site order.
+template <typename R>
+__global__ void object_force_kernel(const R* q, const int8_t* types, const DevObject* objects, int lx, int ly, int lz,
+                                    EnergyConst ec, double L1, const int64_t* sites, long long m, double* out) {
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < m; k += (long long)gridDim.x * blockDim.x) {
+        const long long i = sites[k];
+        double* o = out + 3 * k;
+        if (types[i] > 0) {   // a surface site taken by a later object
+            o[0] = o[1] = o[2] = 0.0;
+            continue;
+        }
+        double parts[5];
+        R dvr[3][5];
+        site_energy(q, types, objects, lx, ly, lz, ec, i, parts, dvr);
+        // the energy density in the order computeEnergyCPU adds it
+        const double e = (((parts[0] + parts[3]) + parts[4]) + parts[2]) + parts[1];
+        double X[5], Y[5], Z[5];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            X[c] = double(dvr[0][c]);
+            Y[c] = double(dvr[1][c]);
+            Z[c] = double(dvr[2][c]);
+        }
+        auto diag = [&](const double* D) {
+            return -2 * L1 * (D[0] * D[0] + 2 * (D[1] * D[1]) + 2 * (D[2] * D[2]) + D[3] * D[3] + 2 * (D[4] * D[4]) +
+                              D[0] * D[3]);
+        };
+        auto off = [&](const double* P, const double* S) {
+            return L1 * (-(S[0] * (2 * P[0] + P[3])) - S[3] * (P[0] + 2 * P[3]) -
+                         4 * (S[1] * P[1] + S[2] * P[2] + S[4] * P[4]));
+        };
+        const double m11 = diag(X) + e, m22 = diag(Y) + e, m33 = diag(Z) + e;
+        const double mxy = off(X, Y), mxz = off(X, Z), myz = off(Y, Z);
+        const int x = int(i % lx), y = int((i / lx) % ly), z = int(i / ((long long)lx * ly));
+        double vx = 0, vy = 0, vz = 0;
+        if (types[wrapi(x - 1, lx) + (long long)lx * (y + (long long)ly * z)] > 0) vx = -1.0;
+        if (types[wrapi(x + 1, lx) + (long long)lx * (y + (long long)ly * z)] > 0) vx = 1.0;
+        if (types[x + (long long)lx * (wrapi(y - 1, ly) + (long long)ly * z)] > 0) vy = -1.0;
+        if (types[x + (long long)lx * (wrapi(y + 1, ly) + (long long)ly * z)] > 0) vy = 1.0;
+        if (types[x + (long long)lx * (y + (long long)ly * wrapi(z - 1, lz))] > 0) vz = -1.0;
+        if (types[x + (long long)lx * (y + (long long)ly * wrapi(z + 1, lz))] > 0) vz = 1.0;
+        // the stress is symmetric: x12 = x21 etc. (scalar3 * Matrix3x3, inc/matrix.h:281-288)
+        o[0] = m11 * vx + mxy * vy + mxz * vz;
+        o[1] = mxy * vx + m22 * vy + myz * vz;
+        o[2] = mxz * vx + myz * vy + m33 * vz;
+    }
+}
+
+// Moves the Q of sites src[k] to dst[k] (k < m) in two passes through tmp, so
+// that every source is read before any destination is written.
+template <typename R>
+__global__ void gather_q_kernel(const R* q, long long n, const int64_t* src, long long m, R* tmp) {
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < m; k += (long long)gridDim.x * blockDim.x)
+#pragma unroll
+        for (int c = 0; c < 5; ++c) tmp[c * m + k] = q[c * n + src[k]];
+}
+
+template <typename R>
+__global__ void scatter_q_kernel(R* q, long long n, const int64_t* dst, long long m, const R* tmp) {
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < m; k += (long long)gridDim.x * blockDim.x)
+#pragma unroll
+        for (int c = 0; c < 5; ++c) q[c * n + dst[k]] = tmp[c * m + k];
 }
 
 // qTensorLatticeModel::computeDefectMeasures (qTensorLatticeModel.cpp:95-133) with
@@ -925,6 +1012,22 @@ void launch_boundary(const BoundaryArgs<R>& a, int mode, int terms, hipStream_t 
     }
 }
 
+template <typename R>
+void launch_object_force(const R* q, const int8_t* types, const DevObject* objects, int lx, int ly, int lz,
+                         const EnergyConst& ec, double L1, const int64_t* sites, long long m, double* out,
+                         hipStream_t s) {
+    const long long blocks = std::max<long long>(1, std::min<long long>((m + 127) / 128, 4096));
+    hipLaunchKernelGGL(object_force_kernel<R>, dim3(unsigned(blocks)), dim3(128), 0, s, q, types, objects, lx, ly, lz,
+                       ec, L1, sites, m, out);
+}
+
+template <typename R>
+void launch_move_q(R* q, long long n, const int64_t* src, const int64_t* dst, long long m, R* tmp, hipStream_t s) {
+    const long long blocks = std::max<long long>(1, std::min<long long>((m + 255) / 256, 4096));
+    hipLaunchKernelGGL(gather_q_kernel<R>, dim3(unsigned(blocks)), dim3(256), 0, s, q, n, src, m, tmp);
+    hipLaunchKernelGGL(scatter_q_kernel<R>, dim3(unsigned(blocks)), dim3(256), 0, s, q, n, dst, m, tmp);
+}
+
 // Work-groups of the fused pass that fit on one CU at once.
 template <typename R>
 int march_blocks_per_cu(int mode, int terms, int tile) {
@@ -979,7 +1082,10 @@ void launch_defects(const R* q, const int8_t* types, long long n, int kind, doub
     template void launch_defects<R>(const R*, const int8_t*, long long, int, double*, hipStream_t);             \
     template int march_blocks_per_cu<R>(int, int, int);                                                          \
     template hipError_t launch_march_batch<R>(const MarchArgs<R>*, const int2*, int, int, int, int, hipStream_t);  \
-    template hipError_t launch_boundary_batch<R>(const BoundaryArgs<R>*, const int2*, int, int, int, hipStream_t);
+    template hipError_t launch_boundary_batch<R>(const BoundaryArgs<R>*, const int2*, int, int, int, hipStream_t);  \
+    template void launch_object_force<R>(const R*, const int8_t*, const DevObject*, int, int, int, const EnergyConst&, \
+                                         double, const int64_t*, long long, double*, hipStream_t);                 \
+    template void launch_move_q<R>(R*, long long, const int64_t*, const int64_t*, long long, R*, hipStream_t);
 
 QMIN_INSTANTIATE(double)
 QMIN_INSTANTIATE(float)

@@ -35,6 +35,12 @@ int launch_energy(const R* q, const int8_t* types, const DevObject* objects, int
                   const EnergyConst& ec, double* partials, hipStream_t s);
 template <typename R>
 void launch_defects(const R* q, const int8_t* types, long long n, int kind, double* out, hipStream_t s);
+template <typename R>
+void launch_object_force(const R* q, const int8_t* types, const DevObject* objects, int lx, int ly, int lz,
+                         const EnergyConst& ec, double L1, const int64_t* sites, long long m, double* out,
+                         hipStream_t s);
+template <typename R>
+void launch_move_q(R* q, long long n, const int64_t* src, const int64_t* dst, long long m, R* tmp, hipStream_t s);
 }  // namespace qmin
 
 using namespace qmin;
@@ -428,14 +434,26 @@ int qm_lattice_add_object(qm_lattice* L, const int32_t* xyz, int64_t n, const do
     L->objects.push_back({btype, W, S0});
     const int j = (int)L->objects.size();
     for (int64_t i : sites) L->types[i] = j;
+    // surface sites in order of discovery, first occurrence kept
+    // (removeDuplicateVectorElements, inc/functions.h:23-35)
+    std::vector<int64_t> surface;
+    std::vector<char> seen(L->n, 0);
     for (int64_t i : sites) {
         const int x = int(i % L->lx), y = int((i / L->lx) % L->ly), z = int(i / ((int64_t)L->lx * L->ly));
         const int64_t nb[6] = {site_of(L, wrapc(x - 1, L->lx), y, z), site_of(L, wrapc(x + 1, L->lx), y, z),
                                site_of(L, x, wrapc(y - 1, L->ly), z), site_of(L, x, wrapc(y + 1, L->ly), z),
                                site_of(L, x, y, wrapc(z - 1, L->lz)), site_of(L, x, y, wrapc(z + 1, L->lz))};
         for (int d = 0; d < 6; ++d)
-            if (L->types[nb[d]] < 1) L->types[nb[d]] = -1;
+            if (L->types[nb[d]] < 1) {
+                L->types[nb[d]] = -1;
+                if (!seen[nb[d]]) {
+                    seen[nb[d]] = 1;
+                    surface.push_back(nb[d]);
+                }
+            }
     }
+    L->objects.back().sites = std::move(sites);
+    L->objects.back().surface = std::move(surface);
     L->typesDirty = true;
     if (int rc = refresh_types(L)) return rc;
     if (obj) *obj = j;
@@ -567,6 +585,75 @@ int qm_lattice_defects(qm_lattice* L, int kind, double* out) {
     HIP_TRY(hipStreamSynchronize(L->stream));
     HIP_TRY(hipFree(d));
     return QM_OK;
+}
+
+// cubicLattice::boundarySites / surfaceSites of object `object` (0-based)
+int qm_lattice_object_sites(const qm_lattice* L, int object, int surface, int64_t* sites, int64_t* count) {
+    if (!L || !count || object < 0 || object >= (int)L->objects.size()) return fail(QM_EINVAL, "bad argument");
+    const std::vector<int64_t>& v = surface ? L->objects[object].surface : L->objects[object].sites;
+    *count = (int64_t)v.size();
+    if (sites) std::memcpy(sites, v.data(), v.size() * sizeof(int64_t));
+    return QM_OK;
+}
+
+namespace {
+template <typename R>
+int move_q(qm_lattice* L, const std::vector<int64_t>& src, const std::vector<int64_t>& dst) {
+    const long long m = (long long)src.size();
+    if (m == 0) return QM_OK;
+    int64_t* d = nullptr;
+    R* tmp = nullptr;
+    HIP_TRY(hipMalloc(&d, 2 * m * sizeof(int64_t)));
+    hipError_t err = hipMalloc(&tmp, 5 * m * sizeof(R));
+    if (err == hipSuccess) err = hipMemcpyAsync(d, src.data(), m * sizeof(int64_t), hipMemcpyHostToDevice, L->stream);
+    if (err == hipSuccess)
+        err = hipMemcpyAsync(d + m, dst.data(), m * sizeof(int64_t), hipMemcpyHostToDevice, L->stream);
+    if (err == hipSuccess) {
+        launch_move_q<R>(state_set<R>(L, L->hCtrl.cur).q, L->n, d, d + m, m, tmp, L->stream);
+        err = hipGetLastError();
+    }
+    if (err == hipSuccess) err = hipStreamSynchronize(L->stream);
+    (void)hipFree(d);
+    if (tmp) (void)hipFree(tmp);
+    if (err != hipSuccess) return fail(QM_EHIP, std::string("moving an object: ") + hipGetErrorString(err));
+    return QM_OK;
+}
+}  // namespace
+
+// cubicLattice::displaceBoundaryObject (cubicLattice.cpp:329-372): per unit
+// step the object's sites and its surface sites, with their Q, move one site
+// along direction (0..5: x-, x+, y-, y+, z-, z+); the old surface sites become
+// bulk first, then the moved sites take the object's type and -1.
+int qm_lattice_displace_object(qm_lattice* L, int object, int direction, int magnitude) {
+    if (!L || object < 0 || object >= (int)L->objects.size() || direction < 0 || direction > 5 || magnitude < 0)
+        return fail(QM_EINVAL, "bad argument");
+    HostObject& ob = L->objects[object];
+    static const int dir[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+    auto moved = [&](int64_t i) {
+        const int x = int(i % L->lx), y = int((i / L->lx) % L->ly), z = int(i / ((int64_t)L->lx * L->ly));
+        return site_of(L, wrapc(x + dir[direction][0], L->lx), wrapc(y + dir[direction][1], L->ly),
+                       wrapc(z + dir[direction][2], L->lz));
+    };
+    for (int mm = 0; mm < magnitude; ++mm) {
+        std::vector<int64_t> src(ob.sites), dst;
+        src.insert(src.end(), ob.surface.begin(), ob.surface.end());
+        dst.reserve(src.size());
+        for (int64_t i : src) dst.push_back(moved(i));
+        for (int64_t i : ob.surface) L->types[i] = 0;
+        const size_t nb = ob.sites.size();
+        for (size_t k = 0; k < nb; ++k) {
+            ob.sites[k] = dst[k];
+            L->types[dst[k]] = object + 1;
+        }
+        for (size_t k = 0; k < ob.surface.size(); ++k) {
+            ob.surface[k] = dst[nb + k];
+            L->types[dst[nb + k]] = -1;
+        }
+        const int rc = L->precision == QM_FP64 ? move_q<double>(L, src, dst) : move_q<float>(L, src, dst);
+        if (rc) return rc;
+    }
+    L->typesDirty = true;
+    return refresh_types(L);
 }
 
 // saveState (multirankSimulation.cpp:630-665): "x y z Q0 Q1 Q2 Q3 Q4 type defect",
@@ -985,8 +1072,7 @@ int qm_ldg_compute_forces(qm_ldg* f, qm_lattice* L) {
     return QM_OK;
 }
 
-int qm_ldg_compute_energy(qm_ldg* f, qm_lattice* L, double* total, double* comps) {
-    if (int rc = prepare(L, f)) return rc;
+static EnergyConst energy_const(const qm_ldg* f, const qm_lattice* L) {
     EnergyConst ec{};
     ec.a = 0.5 * f->A;
     ec.b = f->B / 3.0;
@@ -1009,6 +1095,61 @@ int qm_ldg_compute_energy(qm_ldg* f, qm_lattice* L, double* total, double* comps
     ec.mu0 = f->mu0;
     ec.dchi = f->dchi;
     ec.hvar = f->hvarOn ? L->dHField : nullptr;
+    return ec;
+}
+
+// landauDeGennesLC::computeObjectForces (landauDeGennesLCOtherForces.cpp:147-208)
+// with computeStressTensors (:214-251): the force on object `object` (0-based)
+// from the stress tensor on its surface sites, summed in surface-site order.
+// One-constant models only, as in the reference.
+int qm_ldg_object_force(qm_ldg* f, qm_lattice* L, int object, double* force) {
+    if (!f || !L || !force || object < 0 || object >= (int)L->objects.size()) return fail(QM_EINVAL, "bad argument");
+    if (f->multi)
+        return fail(QM_EINVAL, "object forces need the one-constant model: the reference computes the stress "
+                               "tensor only there (landauDeGennesLCOtherForces.cpp:214-251)");
+    if (int rc = prepare(L, f)) return rc;
+    const std::vector<int64_t>& sur = L->objects[object].surface;
+    const long long m = (long long)sur.size();
+    force[0] = force[1] = force[2] = 0.0;
+    if (m == 0) return QM_OK;
+    const EnergyConst ec = energy_const(f, L);
+    int64_t* dSites = nullptr;
+    double* dOut = nullptr;
+    HIP_TRY(hipMalloc(&dSites, m * sizeof(int64_t)));
+    hipError_t err = hipMalloc(&dOut, 3 * m * sizeof(double));
+    std::vector<double> out(3 * m);
+    if (err == hipSuccess)
+        err = hipMemcpyAsync(dSites, sur.data(), m * sizeof(int64_t), hipMemcpyHostToDevice, L->stream);
+    if (err == hipSuccess) {
+        if (L->precision == QM_FP64)
+            launch_object_force<double>(state_set<double>(L, L->hCtrl.cur).q, L->dTypes, L->dObjects, L->lx, L->ly,
+                                        L->lz, ec, f->L1, dSites, m, dOut, L->stream);
+        else
+            launch_object_force<float>(state_set<float>(L, L->hCtrl.cur).q, L->dTypes, L->dObjects, L->lx, L->ly,
+                                       L->lz, ec, f->L1, dSites, m, dOut, L->stream);
+        err = hipGetLastError();
+    }
+    if (err == hipSuccess)
+        err = hipMemcpyAsync(out.data(), dOut, 3 * m * sizeof(double), hipMemcpyDeviceToHost, L->stream);
+    if (err == hipSuccess) err = hipStreamSynchronize(L->stream);
+    (void)hipFree(dSites);
+    if (dOut) (void)hipFree(dOut);
+    if (err != hipSuccess) return fail(QM_EHIP, std::string("object force: ") + hipGetErrorString(err));
+    double fx = 0, fy = 0, fz = 0;
+    for (long long k = 0; k < m; ++k) {
+        fx = fx + out[3 * k];
+        fy = fy + out[3 * k + 1];
+        fz = fz + out[3 * k + 2];
+    }
+    force[0] = fx;
+    force[1] = fy;
+    force[2] = fz;
+    return QM_OK;
+}
+
+int qm_ldg_compute_energy(qm_ldg* f, qm_lattice* L, double* total, double* comps) {
+    if (int rc = prepare(L, f)) return rc;
+    const EnergyConst ec = energy_const(f, L);
     if (int rc = ensure_partials(L, 5 * (size_t)kMaxEnergyBlocks)) return rc;
     int blocks;
     if (L->precision == QM_FP64)

@@ -18,6 +18,8 @@ using qmin::FireConst;
 struct HostObject {
     int type;
     double W, S0;
+    std::vector<int64_t> sites;     // cubicLattice::boundarySites: creation order
+    std::vector<int64_t> surface;   // cubicLattice::surfaceSites: order of discovery
 };
 
 struct qm_lattice {

@@ -203,11 +203,27 @@ class qTensorLatticeModel : public simpleModel {
     }
     // FP32 <-> FP64 storage (mixed form and FP64 polish); no reference counterpart
     void setPrecision(int precision) { qmin_host::check(qm_lattice_set_precision(lat_, precision), "setPrecision"); }
+    // cubicLattice::displaceBoundaryObject (cubicLattice.cpp:329-372); motionDirection 0..5 = x-, x+, y-, y+, z-, z+
+    void displaceBoundaryObject(int objectIndex, int motionDirection, int magnitude) {
+        qmin_host::check(qm_lattice_displace_object(lat_, objectIndex, motionDirection, magnitude),
+                         "displaceBoundaryObject");
+    }
+    // cubicLattice::boundarySites / surfaceSites of one object
+    std::vector<int64_t> objectSites(int objectIndex, bool surface) {
+        int64_t count = 0;
+        qmin_host::check(qm_lattice_object_sites(lat_, objectIndex, surface ? 1 : 0, nullptr, &count), "objectSites");
+        std::vector<int64_t> v((size_t)count);
+        qmin_host::check(qm_lattice_object_sites(lat_, objectIndex, surface ? 1 : 0, v.data(), &count), "objectSites");
+        return v;
+    }
 
     struct {
         int x, y, z;
     } latticeSites;
     std::vector<scalar> defectMeasures;
+    // cubicLattice::boundaryForce (cubicLattice.h:85-86): per object, the forces
+    // landauDeGennesLC::computeObjectForces has added
+    std::vector<scalar3> boundaryForce;
 
    protected:
     void get(int which, std::vector<double>& a) {
@@ -311,6 +327,18 @@ class landauDeGennesLC {
             std::printf("energy: phase %g distortion %g anchoring %g E %g H %g\n", energyComponents[0],
                         energyComponents[1], energyComponents[2], energyComponents[3], energyComponents[4]);
         return total;
+    }
+
+    // landauDeGennesLC::computeObjectForces (landauDeGennesLCOtherForces.cpp:147-208):
+    // adds the stress-tensor force on the object to the model's boundaryForce and
+    // prints the running total, as the reference does; one-constant only
+    void computeObjectForces(int objectIdx) {
+        double f[3];
+        qmin_host::check(qm_ldg_object_force(f_, lattice(), objectIdx, f), "computeObjectForces");
+        if ((int)model->boundaryForce.size() <= objectIdx) model->boundaryForce.resize(objectIdx + 1, scalar3{0, 0, 0});
+        scalar3& b = model->boundaryForce[objectIdx];
+        b = scalar3{b.x + f[0], b.y + f[1], b.z + f[2]};
+        std::printf("%f\t%f\t%f\n", b.x, b.y, b.z);
     }
 
     qm_ldg* handle() { return f_; }

@@ -80,6 +80,9 @@ _API = {
     "qm_lattice_n_boundary": [_VP, _L],
     "qm_lattice_defects": [_VP, _i, _D],
     "qm_lattice_save_state": [_VP, ctypes.c_char_p],
+    "qm_lattice_object_sites": [_VP, _i, _i, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)],
+    "qm_lattice_displace_object": [_VP, _i, _i, _i],
+    "qm_ldg_object_force": [_VP, _VP, _i, _D],
     "qm_lattice_save_state_ex": [_VP, ctypes.c_char_p, _i, _i],
     "qm_lattice_load_state": [_VP, ctypes.c_char_p],
     "qm_lattice_copy_q": [_VP, _VP],
@@ -311,6 +314,22 @@ class Lattice:
         _check(self.lib.qm_lattice_defects(self.h, kind, _dp(out)))
         return out
 
+    def object_sites(self, obj, surface=False):
+        """Sites of object obj (0-based): its own in creation order, or its
+        surface sites in order of discovery (cubicLattice::boundarySites /
+        surfaceSites)."""
+        count = ctypes.c_int64()
+        _check(self.lib.qm_lattice_object_sites(self.h, obj, int(surface), None, ctypes.byref(count)))
+        out = np.zeros(count.value, np.int64)
+        _check(self.lib.qm_lattice_object_sites(self.h, obj, int(surface),
+                                                out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                                ctypes.byref(count)))
+        return out
+
+    def displace_object(self, obj, direction, magnitude=1):
+        """cubicLattice::displaceBoundaryObject: direction 0..5 = x-, x+, y-, y+, z-, z+."""
+        _check(self.lib.qm_lattice_displace_object(self.h, obj, direction, magnitude))
+
     def save_state(self, path, stride=1, defect_type=0):
         """saveState's text format (one line per site whose coordinates are all
         multiples of stride; last column computeDefectMeasures(defect_type))."""
@@ -371,6 +390,13 @@ class LandauDeGennesLC:
         comps = np.zeros(5)
         _check(self.lib.qm_ldg_compute_energy(self.h, lattice.h, ctypes.byref(tot), _dp(comps)))
         return tot.value, comps
+
+    def object_force(self, lattice, obj):
+        """computeObjectForces: the stress-tensor force on object obj (0-based);
+        one-constant models only, as in the reference."""
+        out = np.zeros(3)
+        _check(self.lib.qm_ldg_object_force(self.h, lattice.h, obj, _dp(out)))
+        return out
 
 
 class FIRE:

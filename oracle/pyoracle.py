@@ -72,6 +72,9 @@ def oracle_lib():
         _sig(lib, "orc_random_init", None, [_VP, _i, _d])
         _sig(lib, "orc_random_uniform_init", None, [_VP, _i, _d])
         _sig(lib, "orc_director_init", None, [_VP, _D])
+        _sig(lib, "orc_object_sites", _ll, [_VP, _i, _i, ctypes.POINTER(ctypes.c_int64)])
+        _sig(lib, "orc_displace_object", _i, [_VP, _i, _i, _i])
+        _sig(lib, "orc_object_force", _i, [_VP, _i, _D])
         _sig(lib, "orc_add_object", _i, [_VP, _I, _i, _D, _i, _d, _d])
         _sig(lib, "orc_wall", _i, [_VP, _i, _i, _i, _d, _d])
         _sig(lib, "orc_sphere", _i, [_VP, _d, _d, _d, _d, _i, _d, _d])
@@ -122,6 +125,8 @@ def ref_lib():
         _sig(lib, "qmref_model_neighbors", None, [_VP, _I])
         _sig(lib, "qmref_model_random_init", None, [_VP, _i, _d])
         _sig(lib, "qmref_model_random_init_aligned", None, [_VP, _i, _d])
+        _sig(lib, "qmref_model_object_sites", _i, [_VP, _i, _i, _I])
+        _sig(lib, "qmref_model_displace_object", None, [_VP, _i, _i, _i])
         _sig(lib, "qmref_normalize_director", None, [_D])
         _sig(lib, "qmref_model_add_object", None, [_VP, _I, _i, _i, _d, _d])
         _sig(lib, "qmref_model_get_q", None, [_VP, _D])
@@ -228,6 +233,21 @@ class OracleSystem:
         n4 = np.ascontiguousarray(n4, np.float64)
         assert n4.shape == (self.n, 4)
         self.lib.orc_director_init(self.h, dptr(n4))
+
+    def object_sites(self, obj, surface=False):
+        count = self.lib.orc_object_sites(self.h, obj, int(surface), None)
+        assert count >= 0
+        out = np.zeros(count, np.int64)
+        self.lib.orc_object_sites(self.h, obj, int(surface), out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+        return out
+
+    def displace_object(self, obj, direction, magnitude):
+        assert self.lib.orc_displace_object(self.h, obj, direction, magnitude) == 0
+
+    def object_force(self, obj):
+        out = np.zeros(3)
+        assert self.lib.orc_object_force(self.h, obj, dptr(out)) == 0
+        return out
 
     def add_object(self, xyz, qb, btype, W, S0):
         xyz = np.ascontiguousarray(xyz, np.int32)
@@ -336,6 +356,15 @@ class RefModel:
 
     def random_init_aligned(self, seed, S0):
         self.lib.qmref_model_random_init_aligned(self.h, seed, S0)
+
+    def object_sites(self, obj, surface=False):
+        count = self.lib.qmref_model_object_sites(self.h, obj, int(surface), None)
+        out = np.zeros(count, np.int32)
+        self.lib.qmref_model_object_sites(self.h, obj, int(surface), iptr(out))
+        return out.astype(np.int64)
+
+    def displace_object(self, obj, direction, magnitude):
+        self.lib.qmref_model_displace_object(self.h, obj, direction, magnitude)
 
     def add_object(self, sites, btype, P1, P2):
         sites = np.ascontiguousarray(sites, np.int32)

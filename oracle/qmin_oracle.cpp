@@ -5,6 +5,7 @@
 #include "qmin_oracle.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <random>
@@ -32,6 +33,9 @@ struct orc_system {
     std::vector<int32_t> types, nbr;
     std::vector<double> q, v, f;
     std::vector<BoundaryObj> objects;
+    // per object, as cubicLattice keeps them (cubicLattice.h:78-86): its sites in
+    // creation order, and its surface sites in order of discovery
+    std::vector<std::vector<int64_t>> objSites, objSurface;
     double A = -1, B = 0, C = 0, L1 = 0, L2 = 0, L3 = 0, L4 = 0, L6 = 0;
     int multi = 0, semantics = 0;
     int eOn = 0, hOn = 0;
@@ -635,12 +639,66 @@ int orc_add_object(orc_system* s, const int32_t* xyz, int n, const double* qb, i
     s->objects.push_back({btype, W, S0});
     int j = (int)s->objects.size();
     for (int64_t site : sites) s->types[site] = j;
+    // cubicLattice::createBoundaryObject (cubicLattice.cpp:282-306): every
+    // non-object neighbour becomes a surface site, duplicates removed keeping
+    // the first occurrence (removeDuplicateVectorElements, inc/functions.h:23-35)
+    std::vector<int64_t> surface;
+    std::vector<char> seen(s->n, 0);
     for (int64_t site : sites)
         for (int d = 0; d < 6; ++d) {
             int32_t nb = s->nbr[6 * site + d];
-            if (s->types[nb] < 1) s->types[nb] = -1;
+            if (s->types[nb] < 1) {
+                s->types[nb] = -1;
+                if (!seen[nb]) {
+                    seen[nb] = 1;
+                    surface.push_back(nb);
+                }
+            }
         }
+    s->objSites.push_back(sites);
+    s->objSurface.push_back(surface);
     return j;
+}
+
+// cubicLattice::boundarySites / surfaceSites of object obj (0-based); returns the count
+int64_t orc_object_sites(orc_system* s, int obj, int surface, int64_t* out) {
+    if (obj < 0 || obj >= (int)s->objSites.size()) return -1;
+    const std::vector<int64_t>& v = surface ? s->objSurface[obj] : s->objSites[obj];
+    if (out) std::memcpy(out, v.data(), v.size() * sizeof(int64_t));
+    return (int64_t)v.size();
+}
+
+// cubicLattice::displaceBoundaryObject (cubicLattice.cpp:329-372), CPU branch:
+// per unit step, the object's and its surface's Q move one site along
+// direction (neighbour order x-, x+, y-, y+, z-, z+); the old surface sites
+// become bulk (type 0) first, then the moved sites take the object's type and -1.
+int orc_displace_object(orc_system* s, int obj, int direction, int magnitude) {
+    if (obj < 0 || obj >= (int)s->objSites.size() || direction < 0 || direction > 5) return -1;
+    std::vector<int64_t>& bs = s->objSites[obj];
+    std::vector<int64_t>& ss = s->objSurface[obj];
+    for (int mm = 0; mm < magnitude; ++mm) {
+        std::vector<std::pair<int64_t, std::array<double, 5>>> b1(bs.size()), b2(ss.size());
+        for (size_t k = 0; k < bs.size(); ++k) {
+            b1[k].first = s->nbr[6 * bs[k] + direction];
+            std::memcpy(b1[k].second.data(), &s->q[5 * bs[k]], 5 * sizeof(double));
+        }
+        for (size_t k = 0; k < ss.size(); ++k) {
+            b2[k].first = s->nbr[6 * ss[k] + direction];
+            std::memcpy(b2[k].second.data(), &s->q[5 * ss[k]], 5 * sizeof(double));
+            s->types[ss[k]] = 0;
+        }
+        for (size_t k = 0; k < bs.size(); ++k) {
+            bs[k] = b1[k].first;
+            std::memcpy(&s->q[5 * bs[k]], b1[k].second.data(), 5 * sizeof(double));
+            s->types[bs[k]] = obj + 1;
+        }
+        for (size_t k = 0; k < ss.size(); ++k) {
+            ss[k] = b2[k].first;
+            std::memcpy(&s->q[5 * ss[k]], b2[k].second.data(), 5 * sizeof(double));
+            s->types[ss[k]] = -1;
+        }
+    }
+    return 0;
 }
 
 // multirankSimulation::createWall (multirankSimulationBoundaries.cpp:101-167)
@@ -790,6 +848,81 @@ double orc_energy(orc_system* s, const double* q, double* comps) {
         comps[0] = phase; comps[1] = distortion; comps[2] = anchoring; comps[3] = eF; comps[4] = hF;
     }
     return phase + distortion + anchoring + eF + hF;
+}
+
+// landauDeGennesLC::computeObjectForces (landauDeGennesLCOtherForces.cpp:147-208)
+// with computeStressTensors (:214-251; one-constant only, as there): over the
+// object's surface sites, the stress tensor from the first derivatives
+// (computeFirstDerivatives rules) and L1, plus the site's energy density
+// (computeEnergyCPU: phase, fields, anchoring, distortion) on the diagonal,
+// applied to the site's surface-area vector (+-1 per axis towards an object
+// neighbour, the upper neighbour taking precedence); summed in surface-site
+// order.  Returns -1 unless the model is one-constant.
+int orc_object_force(orc_system* s, int obj, double* out) {
+    if (obj < 0 || obj >= (int)s->objSurface.size() || s->multi) return -1;
+    const double* q = s->q.data();
+    const double a = 0.5 * s->A, b = s->B / 3.0, c = 0.25 * s->C, L1 = s->L1;
+    double fx = 0, fy = 0, fz = 0;
+    for (int64_t i : s->objSurface[obj]) {
+        const int t = s->types[i];
+        if (t > 0) continue;   // a surface site taken by a later object
+        const double* qc = q + 5 * i;
+        const int32_t* nb = &s->nbr[6 * i];
+        // energy density of the site (computeEnergyCPU)
+        double e = 0;
+        const double t2 = trQ2(qc);
+        e += a * t2 + b * trQ3(qc) + c * t2 * t2;
+        if (s->eOn) e += fieldEnergy(s->E, s->eps, s->deps, s->eps0, qc);
+        if (s->hOn) e += fieldEnergy(s->H, s->chi, s->dchi, s->mu0, qc);
+        if (s->hvarOn) e += fieldEnergy(&s->hvar[3 * i], s->chi, s->dchi, s->mu0, qc);
+        double d[3][5];
+        for (int ax = 0; ax < 3; ++ax)
+            for (int k = 0; k < 5; ++k) d[ax][k] = 0.5 * (q[5 * (int64_t)nb[2 * ax + 1] + k] - q[5 * (int64_t)nb[2 * ax] + k]);
+        double anch = 0;
+        if (t < 0)
+            for (int dd = 0; dd < 6; ++dd) {
+                const int tn = s->types[nb[dd]];
+                if (tn <= 0) continue;
+                anch += anchoringEnergy(s->objects[tn - 1], qc, q + 5 * (int64_t)nb[dd]);
+                const int ax = dd / 2;
+                const double* other = q + 5 * (int64_t)nb[dd ^ 1];
+                for (int k = 0; k < 5; ++k) d[ax][k] = (dd & 1) ? (qc[k] - other[k]) : (other[k] - qc[k]);
+            }
+        e += anch;
+        double et[5];
+        distortionTerms(qc, d[0], d[1], d[2], et);
+        e += L1 * et[0];
+        // stress (one-constant form), fd = [dx q0..4, dy q0..4, dz q0..4]
+        const double* X = d[0];
+        const double* Y = d[1];
+        const double* Z = d[2];
+        auto diag = [&](const double* D) {
+            return -2 * L1 * (D[0] * D[0] + 2 * (D[1] * D[1]) + 2 * (D[2] * D[2]) + D[3] * D[3] + 2 * (D[4] * D[4]) +
+                              D[0] * D[3]);
+        };
+        auto off = [&](const double* P, const double* R) {
+            return L1 * (-(R[0] * (2 * P[0] + P[3])) - R[3] * (P[0] + 2 * P[3]) -
+                         4 * (R[1] * P[1] + R[2] * P[2] + R[4] * P[4]));
+        };
+        const double m11 = diag(X) + e, m22 = diag(Y) + e, m33 = diag(Z) + e;
+        const double m12 = off(X, Y), m13 = off(X, Z), m21 = off(X, Y), m23 = off(Y, Z), m31 = off(X, Z),
+                     m32 = off(Y, Z);
+        double vx = 0, vy = 0, vz = 0;
+        if (s->types[nb[0]] > 0) vx = -1.0;
+        if (s->types[nb[1]] > 0) vx = 1.0;
+        if (s->types[nb[2]] > 0) vy = -1.0;
+        if (s->types[nb[3]] > 0) vy = 1.0;
+        if (s->types[nb[4]] > 0) vz = -1.0;
+        if (s->types[nb[5]] > 0) vz = 1.0;
+        // scalar3 * Matrix3x3 (inc/matrix.h:281-288)
+        fx = fx + (m11 * vx + m12 * vy + m13 * vz);
+        fy = fy + (m21 * vx + m22 * vy + m23 * vz);
+        fz = fz + (m31 * vx + m32 * vy + m33 * vz);
+    }
+    out[0] = fx;
+    out[1] = fy;
+    out[2] = fz;
+    return 0;
 }
 
 double orc_energy_cb(void* ctx, int n, const double* q) {

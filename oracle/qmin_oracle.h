@@ -35,6 +35,9 @@ void orc_set_q(orc_system* s, const double* in);
 void orc_random_init(orc_system* s, int seed, double S0);
 void orc_random_uniform_init(orc_system* s, int seed, double S0);
 void orc_director_init(orc_system* s, const double* n4);
+int64_t orc_object_sites(orc_system* s, int obj, int surface, int64_t* out);
+int orc_displace_object(orc_system* s, int obj, int direction, int magnitude);
+int orc_object_force(orc_system* s, int obj, double* out);
 // btype 0 homeotropic, 1 degenerate planar; sites are (x,y,z) triples, qb is [n][5]
 int orc_add_object(orc_system* s, const int32_t* xyz, int n, const double* qb, int btype, double W, double S0);
 int orc_wall(orc_system* s, int xyz, int plane, int btype, double W, double S0);

@@ -258,6 +258,27 @@ void qmref_model_random_init_aligned(void* h, int seed, double S0) {
     m->setNematicQTensorRandomly(noise, S0, true);
 }
 
+// cubicLattice::boundarySites / surfaceSites of object obj; returns the count
+int qmref_model_object_sites(void* h, int obj, int surface, int* out) {
+    auto& m = static_cast<RefModel*>(h)->model;
+    GPUArray<int>& a = surface ? m->surfaceSites[obj] : m->boundarySites[obj];
+    const int count = a.getNumElements();
+    if (out) {
+        ArrayHandle<int> v(a, access_location::host, access_mode::read);
+        for (int i = 0; i < count; ++i) out[i] = v.data[i];
+    }
+    return count;
+}
+
+void qmref_model_displace_object(void* h, int obj, int direction, int magnitude) {
+    auto& m = static_cast<RefModel*>(h)->model;
+    {
+        Quiet q;
+        m->fillNeighborLists(0);   // the move reads the neighbour table
+    }
+    m->displaceBoundaryObject(obj, direction, magnitude);
+}
+
 // normalizeDirector (inc/functions.h), in place on n[3]
 void qmref_normalize_director(double* n) {
     scalar3 v;

@@ -205,6 +205,35 @@ def test_random_uniform_and_function_inits_bit_exact():
     assert np.array_equal(g.q(), o.q())
 
 
+def test_object_sites_displacement_and_force_match_oracle():
+    """Object and surface site lists, displaceBoundaryObject (types and Q, bit
+    for bit) and the stress-tensor force on an object (computeObjectForces) on
+    the device, against the oracle; multi-constant models are refused, as the
+    reference computes the stress only for one constant."""
+    shape = (16, 14, 12)
+    o, g, f = build_pair(shape, ONE, 1, True)
+    for k in range(3):
+        assert np.array_equal(g.object_sites(k), o.object_sites(k))
+        assert np.array_equal(g.object_sites(k, surface=True), o.object_sites(k, surface=True))
+    for k in range(3):
+        np.testing.assert_allclose(f.object_force(g, k), o.object_force(k), rtol=1e-12, atol=1e-12)
+    for obj, direction, magnitude in ((2, 1, 2), (2, 4, 1), (1, 5, 1)):
+        g.displace_object(obj, direction, magnitude)
+        o.displace_object(obj, direction, magnitude)
+        assert np.array_equal(g.types(), o.types())
+        assert np.array_equal(g.q(), o.q(), equal_nan=True)
+        assert np.array_equal(g.object_sites(obj, surface=True), o.object_sites(obj, surface=True))
+    np.testing.assert_allclose(f.object_force(g, 2), o.object_force(2), rtol=1e-12, atol=1e-12)
+    # after FIRE the force follows the oracle's trajectory
+    m = qmin.FIRE(max_iter=20)
+    m.minimize(f, g)
+    o.fire(max_iter=20)
+    np.testing.assert_allclose(f.object_force(g, 2), o.object_force(2), rtol=1e-9, atol=1e-11)
+    multi = qmin.LandauDeGennesLC(A, B, C, *MULTI, semantics=1)
+    with pytest.raises(qmin.QminError):
+        multi.object_force(g, 2)
+
+
 def test_fire_continues_across_calls():
     """Two minimize calls of 20 iterations equal one of 40 (state persists on the device)."""
     o, g, f = build_pair((18, 14, 12), MULTI, 0, True)

@@ -331,3 +331,71 @@ def test_varying_field_oracle_reduces_to_uniform_and_is_local():
     f1 = o.forces()
     changed = np.where(np.abs(f1 - f0).max(axis=1) > 0)[0]
     assert changed.tolist() == [17]
+
+
+def _object_scene(phase):
+    A, B, C, S0 = phase
+    o = po.OracleSystem(10, 9, 8)
+    o.random_init(77, S0)
+    o.wall(2, 0, 1, 5.3, S0)
+    o.sphere((5.0, 4.5, 4.0), 2.2, 0, 5.3, S0)
+    o.sphere((1.5, 6.0, 5.5), 1.6, 1, 2.0, S0)
+    return o
+
+
+@live_ref
+def test_object_sites_and_displacement_match_reference(phase):
+    """cubicLattice::createBoundaryObject's surface sites (discovery order,
+    duplicates dropped) and displaceBoundaryObject (types and Q) of the compiled
+    reference, bit for bit, for moves along several axes and lengths."""
+    A, B, C, S0 = phase
+    o = _object_scene(phase)
+    r = po.RefModel(10, 9, 8)
+    for k in range(3):
+        r.add_object(o.object_sites(k).astype(np.int32), 1 if k == 0 else 0, 5.3, S0)
+    r.set_q(o.q())
+    assert np.array_equal(r.types(), o.types())
+    for k in range(3):
+        assert np.array_equal(r.object_sites(k), o.object_sites(k))
+        assert np.array_equal(r.object_sites(k, surface=True), o.object_sites(k, surface=True))
+    for obj, direction, magnitude in ((1, 1, 2), (1, 4, 1), (2, 3, 3), (0, 5, 1), (1, 0, 1)):
+        r.displace_object(obj, direction, magnitude)
+        o.displace_object(obj, direction, magnitude)
+        assert np.array_equal(r.types(), o.types()), (obj, direction, magnitude)
+        assert np.array_equal(r.q(), o.q(), equal_nan=True), (obj, direction, magnitude)
+        assert np.array_equal(r.object_sites(obj, surface=True), o.object_sites(obj, surface=True))
+
+
+def test_object_force_properties(phase):
+    """computeObjectForces on the oracle (the reference force class needs MPI and
+    is not compiled here, so the stress formula is pinned by properties): a
+    configuration with mirror symmetries has no net force; translating the whole
+    scene leaves the force unchanged; a homeotropic sphere next to a planar
+    wall, relaxed, is pushed along z only (the scene's x and y mirror symmetry)."""
+    A, B, C, S0 = phase
+
+    def scene(shift=(0, 0, 0), relax=0):
+        o = po.OracleSystem(12, 12, 12)
+        o.uniform_init((0.0, 0.0, 1.0), S0) if hasattr(o, "uniform_init") else o.director_init(
+            np.tile([0.0, 0.0, 1.0, S0], (o.n, 1)))
+        o.sphere((6.0 + shift[0], 6.0 + shift[1], 6.0 + shift[2]), 3.0, 0, 5.3, S0)
+        o.set_params(A, B, C, 2.32, semantics=1)
+        if relax:
+            o.fire(max_iter=relax)
+        return o
+
+    f0 = scene().object_force(0)
+    assert np.abs(f0).max() < 1e-12
+    f1 = scene(relax=40).object_force(0)
+    f2 = scene(shift=(2, 3, 1), relax=40).object_force(0)
+    np.testing.assert_allclose(f2, f1, rtol=1e-10, atol=1e-12)
+    assert np.abs(f1[:2]).max() < 1e-10
+    # a planar wall below the sphere breaks the z mirror: a nonzero push along z
+    o = po.OracleSystem(12, 12, 16)
+    o.director_init(np.tile([0.0, 0.0, 1.0, S0], (o.n, 1)))
+    o.wall(2, 0, 1, 5.3, S0)
+    o.sphere((6.0, 6.0, 6.5), 3.0, 0, 5.3, S0)
+    o.set_params(A, B, C, 2.32, semantics=1)
+    o.fire(max_iter=60)
+    f = o.object_force(1)
+    assert np.abs(f[:2]).max() < 1e-9 * abs(f[2]) + 1e-13 and abs(f[2]) > 1e-6
