@@ -1,7 +1,7 @@
 """Train the warm-start network on converged lattices from the ensemble
 minimizer and measure what it buys on held-out geometries.
 
-  python open-qmin_amd/qmin/warmstart_experiment.py [--train 512] [--epochs 6]
+  python open-qmin_amd/qmin/warmstart_experiment.py [--train 4096] [--epochs 8]
          [--weights out.npz] [--load weights.npz] [--skip-256]
 
 Training set: `--train` 64^3 lattices (half with z walls of either anchoring,
@@ -87,8 +87,8 @@ class Geometry:
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--train", type=int, default=512)
-    ap.add_argument("--epochs", type=int, default=6)
+    ap.add_argument("--train", type=int, default=4096)
+    ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--lr", type=float, default=5e-4)
     ap.add_argument("--weights", default="")
     ap.add_argument("--load", default="")
