@@ -843,11 +843,12 @@ static int upload_weights(qm_lattice* L, const qm_ldg* f) {
 
 // Tile shape of the fused pass: the one set on the lattice, else the measured
 // better default: 32 x 16 on lattices at least 128 sites wide in x and y (256^3:
-// 2% faster than 32 x 8 in FP64, 6% faster than 64 x 4 in FP32); on smaller ones
-// 32 x 8 for FP64 and 64 x 4 for FP32, which waste fewer threads on partial tiles.
+// 2% faster than 32 x 8 in FP64, 6% faster than 64 x 4 in FP32); on smaller ones,
+// and with the L6 term (whose pass needs more than the 128 registers the 32 x 16
+// tile allows), 32 x 8 for FP64 and 64 x 4 for FP32.
 int qmin::active_tile(const qm_lattice* L) {
     if (L->tileOverride >= 0) return L->tileOverride;
-    if (L->lx >= 128 && L->ly >= 128) return kTile32x16;
+    if (L->terms != kMultiL6 && L->lx >= 128 && L->ly >= 128) return kTile32x16;
     return L->precision == QM_FP64 ? kTile32x8 : kTile64x4;
 }
 
@@ -1043,6 +1044,7 @@ int qm_lattice_autotune(qm_lattice* L, qm_ldg* f, int* tileOut, int* zchunkOut, 
 
 int qmin::prepare(qm_lattice* L, const qm_ldg* f) {
     if (!L || !f) return fail(QM_EINVAL, "null handle");
+    L->terms = terms_of(f);
     if (f->hvarOn && !L->dHField)
         return fail(QM_ESTATE, "the force uses a spatially varying field but the lattice has none");
     if (int rc = refresh_types(L)) return rc;

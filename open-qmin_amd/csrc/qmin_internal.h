@@ -48,6 +48,7 @@ struct qm_lattice {
     int zchunk = 16;          // z extent one work-group marches through
     int zchunkOverride = 0;   // fixed z chunk (0 = choose from occupancy)
     int tileOverride = -1;    // tile shape of the fused pass (-1 = by precision)
+    int terms = 0;            // distortion terms of the force last prepared on this lattice
     double* dHField = nullptr;  // spatially varying H: planes hx, hy, hz of n (FP64)
     int numCUs = 0;
 };

@@ -257,11 +257,12 @@ def l6_operand(k):
 
 def gen_fast_l6(entries):
     """L6 bulk operator reading its operands from LDS as it goes: terms grouped by
-    the first factor (a Q value), with scheduling fences between groups so that
-    only the five accumulators and a few operands are live at a time."""
+    the first factor x_a (a Q value) and, within a group, by output component c,
+    as acc[c] += x_a * (sum_b coef x_b), one fused multiply-add per term; scheduling
+    fences between groups keep only the accumulators and a few operands live."""
     groups = {}
     for comp, coef, a, b in entries:
-        groups.setdefault(a, []).append((comp, coef, b))
+        groups.setdefault(a, {}).setdefault(comp, []).append((coef, b))
     lines = ["template <int HX, int HP, typename R>",
              "__device__ __forceinline__ void ldg_fast_l6(const R (*q)[5][HP], int sm, int sc, int sp, int h,"
              " const R* qc, R* acc) {"]
@@ -270,8 +271,14 @@ def gen_fast_l6(entries):
             lines.append("    __builtin_amdgcn_sched_barrier(0);")
         lines.append("    {")
         lines.append("        const R xa = %s;" % l6_operand(a))
-        for comp, coef, b in groups[a]:
-            lines.append("        acc[%d] += %s * (xa * (%s));" % (comp, coef_literal(coef), l6_operand(b)))
+        for comp in sorted(groups[a]):
+            terms = groups[a][comp]
+            coef, b = terms[0]
+            lines.append("        R t%d = %s * (%s);" % (comp, coef_literal(coef), l6_operand(b)))
+            for coef, b in terms[1:]:
+                lines.append("        t%d += %s * (%s);" % (comp, coef_literal(coef), l6_operand(b)))
+        for comp in sorted(groups[a]):
+            lines.append("        acc[%d] += xa * t%d;" % (comp, comp))
         lines.append("    }")
     lines.append("}")
     return lines
