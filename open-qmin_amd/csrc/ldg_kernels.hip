@@ -378,8 +378,16 @@ __device__ __forceinline__ int xcd_logical(int b, int nb) {
     return xcd * qb + min(xcd, rb) + (b >> 3);
 }
 
+// Minimum waves per SIMD of the fused pass: the FP64 FIRE pass with L6 runs
+// three waves at a cap of 168 VGPRs (a few bytes spilled) 20% faster than the
+// two its uncapped register count allows (7.66 -> 6.01 ms at 384^3); its FP32
+// form fits three waves at 151 VGPRs, and forcing four (128, spilling) made it
+// slower (3.10 -> 3.28 ms). 1 leaves a kernel as the compiler sizes it.
+template <typename R, int MODE, int TERMS>
+constexpr int kMarchMinWaves = (TERMS == kMultiL6 && MODE == kFire && sizeof(R) == 8) ? 3 : 1;
+
 template <typename R, int MODE, int TERMS, int TILE>
-__global__ void __launch_bounds__(kTileThreads[TILE]) march_kernel(MarchArgs<R> A) {
+__global__ void __launch_bounds__(kTileThreads[TILE], (kMarchMinWaves<R, MODE, TERMS>)) march_kernel(MarchArgs<R> A) {
     march_body<R, MODE, TERMS, TILE>(A, xcd_logical(blockIdx.x, A.nblocks), blockIdx.x);
 }
 
