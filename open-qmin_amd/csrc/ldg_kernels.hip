@@ -85,36 +85,6 @@ __device__ __forceinline__ void nbr_derivative(int tn, int tu, int td, const R* 
     }
 }
 
-// Central-difference derivatives of the six face neighbours of the LDS centre
-// h (slot sc), transverse axes only, in the layout of the stencil tables'
-// derivative inputs x[35 + 15 d + 5 axis + c] (along-axis entries zero).
-template <int HX, int HP, typename R>
-__device__ __forceinline__ void central_nbr_derivatives(const R (*q)[5][HP], int sm, int sc, int sp, int h, R* x) {
-#pragma unroll
-    for (int d = 0; d < 6; ++d) {
-        const int axis = d / 2;
-        const int sn = d == 4 ? sm : (d == 5 ? sp : sc);
-        const int hn = d == 0 ? h - 1 : d == 1 ? h + 1 : d == 2 ? h - HX : d == 3 ? h + HX : h;
-#pragma unroll
-        for (int ax = 0; ax < 3; ++ax) {
-            R* out = x + 35 + 15 * d + 5 * ax;
-            if (ax == axis) {
-#pragma unroll
-                for (int c = 0; c < 5; ++c) out[c] = R(0);
-                continue;
-            }
-            int su = sn, sd = sn, hu = hn, hd = hn;
-            if (ax == 0) { hu = hn + 1; hd = hn - 1; }
-            if (ax == 1) { hu = hn + HX; hd = hn - HX; }
-            if (ax == 2) { su = sp; sd = sm; }
-#pragma unroll
-            for (int c = 0; c < 5; ++c) out[c] = R(0.5) * (q[su][c][hu] - q[sd][c][hd]);
-        }
-#pragma unroll
-        for (int c = 0; c < 5; ++c) x[5 + 5 * d + c] = q[sn][c][hn];
-    }
-}
-
 // Raw force (phase - distortion) of a site whose neighbours all have central
 // derivatives: the bulk L1..L4 forms composed with those differences into one
 // linear 19-point operator (weights W), plus the quadratic L6 form.
@@ -123,7 +93,7 @@ __device__ __forceinline__ void central_nbr_derivatives(const R (*q)[5][HP], int
 template <typename R>
 using ConstPtr = const R __attribute__((address_space(4)))*;
 
-template <typename R, int TERMS, int HX, int HP, bool L6_FROM_LDS>
+template <typename R, int TERMS, int HX, int HP>
 __device__ __forceinline__ void fast_raw_force(const ForceConst& fc, ConstPtr<R> W, const R (*q)[5][HP],
                                                int sm, int sc, int sp, int h, const R* qc, R* F) {
 #pragma unroll
@@ -143,18 +113,7 @@ __device__ __forceinline__ void fast_raw_force(const ForceConst& fc, ConstPtr<R>
     }
     if constexpr (TERMS == kMultiL6) {
         R o6[5] = {R(0), R(0), R(0), R(0), R(0)};
-        if constexpr (L6_FROM_LDS) {
-            // operands read from LDS as each product needs them: fewer live
-            // registers (higher occupancy) for more LDS reads; the better trade
-            // in the FIRE pass, whose memory traffic hides the LDS time
-            ldg_fast_l6<HX, HP>(q, sm, sc, sp, h, qc, o6);
-        } else {
-            R x[125];
-#pragma unroll
-            for (int c = 0; c < 5; ++c) x[c] = qc[c];
-            central_nbr_derivatives<HX, HP>(q, sm, sc, sp, h, x);
-            ldg_bulk_l6(x, o6);
-        }
+        ldg_fast_l6<HX, HP>(q, sm, sc, sp, h, qc, o6);
 #pragma unroll
         for (int c = 0; c < 5; ++c) spatial[c] += R(fc.L6) * o6[c];
     }
@@ -316,8 +275,8 @@ __device__ __forceinline__ void march_body(const MarchArgs<R>& A, const int logi
                     // inside the loop instead of being hoisted and spilled
                     uint64_t wa = reinterpret_cast<uint64_t>(A.W);
                     asm volatile("" : "+s"(wa));
-                    fast_raw_force<R, TERMS, kHX, kHP, MODE == kFire>(A.fc, reinterpret_cast<ConstPtr<R>>(wa), sQ, sm,
-                                                                      sc, sp, hOwn, q, F);
+                    fast_raw_force<R, TERMS, kHX, kHP>(A.fc, reinterpret_cast<ConstPtr<R>>(wa), sQ, sm, sc, sp,
+                                                       hOwn, q, F);
                     if (A.hvar) {
                         R fv[5];
                         varying_field_force(A.hvar, n, i, A.hvarP, fv);
@@ -378,16 +337,8 @@ __device__ __forceinline__ int xcd_logical(int b, int nb) {
     return xcd * qb + min(xcd, rb) + (b >> 3);
 }
 
-// Minimum waves per SIMD of the fused pass: the FP64 FIRE pass with L6 runs
-// three waves at a cap of 168 VGPRs (a few bytes spilled) 20% faster than the
-// two its uncapped register count allows (7.66 -> 6.01 ms at 384^3); its FP32
-// form fits three waves at 151 VGPRs, and forcing four (128, spilling) made it
-// slower (3.10 -> 3.28 ms). 1 leaves a kernel as the compiler sizes it.
-template <typename R, int MODE, int TERMS>
-constexpr int kMarchMinWaves = (TERMS == kMultiL6 && MODE == kFire && sizeof(R) == 8) ? 3 : 1;
-
 template <typename R, int MODE, int TERMS, int TILE>
-__global__ void __launch_bounds__(kTileThreads[TILE], (kMarchMinWaves<R, MODE, TERMS>)) march_kernel(MarchArgs<R> A) {
+__global__ void __launch_bounds__(kTileThreads[TILE]) march_kernel(MarchArgs<R> A) {
     march_body<R, MODE, TERMS, TILE>(A, xcd_logical(blockIdx.x, A.nblocks), blockIdx.x);
 }
 

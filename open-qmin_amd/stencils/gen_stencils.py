@@ -11,7 +11,6 @@ Output: ../csrc/gen/ldg_stencils.inc with
   ldg_bulk_lin_q(q, acc)                  centre-site part of the bulk L2/L3/L4 forms
   ldg_bulk_lin_n<d>(nq, dA, dB, acc)      part of neighbour d (its Q and its two
                                           transverse derivative vectors A < B)
-  ldg_bulk_l6(x, out)                     bulk L6 form (quadratic)
   ldg_boundary_lin(bcase, x, acc)         the 27 boundary-case L2/L3/L4 forms
   ldg_boundary_l6(bcase, x, out)          the 27 boundary-case L6 forms
 
@@ -26,7 +25,11 @@ differences into one linear operator on the 19-point neighbourhood (centre,
 
   ldg_fast_lin_a / _b(W, q, sm, sc, sp, h, acc)   device: acc[r] += W[k] Q_c(p)
   ../csrc/gen/ldg_fast_host.inc                   host: C1..C4 per entry k
-variant a covers L1..L3 (L4 = 0), variant b L1..L4.
+variant a covers L1..L3 (L4 = 0), variant b L1..L4.  The bulk L6 form, a
+quadratic form in the same central differences, becomes one over the 19 point
+values:
+
+  ldg_fast_l6(q, sm, sc, sp, h, qc, acc)          device: acc[c] += x_a (sum_b coef x_b)
 """
 import json
 import os
@@ -227,58 +230,110 @@ def gen_fast(tables):
     return dev, host
 
 
-L6_FENCE_EVERY = int(os.environ.get("QMIN_L6_FENCE_EVERY", "1"))
+L6_FACES = ((-1, 0, 0), (1, 0, 0), (0, -1, 0), (0, 1, 0), (0, 0, -1), (0, 0, 1))
 
 
-def l6_operand(k):
-    """Expression of stencil input k read straight from the LDS ring (q[slot][c][pos])
-    or the site's own registers (qc): input layout as the tables, x[0..4] the site,
-    x[5 + 5 d + c] neighbour d's Q, x[35 + 15 d + 5 axis + c] its central derivative
-    along a transverse axis; neighbours in the order xd, xu, yd, yu, zd, zu."""
-    pos = [("sc", "h - 1"), ("sc", "h + 1"), ("sc", "h - HX"), ("sc", "h + HX"), ("sm", "h"), ("sp", "h")]
+def l6_point_form(k):
+    """Stencil input k as a linear form over the 19-point neighbourhood:
+    {(point, component): weight}; a neighbour's transverse derivative is the
+    central difference 0.5 (Q(up) - Q(down)) of two edge points."""
     if k < 5:
-        return "qc[%d]" % k
+        return {((0, 0, 0), k): 1.0}
     if k < 35:
         d, c = divmod(k - 5, 5)
-        slot, at = pos[d]
-        return "q[%s][%d][%s]" % (slot, c, at)
+        return {(L6_FACES[d], c): 1.0}
     d, rest = divmod(k - 35, 15)
     ax, c = divmod(rest, 5)
     assert ax != d // 2, "along-axis neighbour derivatives are not inputs"
-    slot, at = pos[d]
-    if ax == 0:
-        up, dn = (slot, "%s + 1" % at), (slot, "%s - 1" % at)
-    elif ax == 1:
-        up, dn = (slot, "%s + HX" % at), (slot, "%s - HX" % at)
-    else:
-        up, dn = ("sp", at), ("sm", at)
-    return "R(0.5) * (q[%s][%d][%s] - q[%s][%d][%s])" % (up[0], c, up[1], dn[0], c, dn[1])
+    up, dn = list(L6_FACES[d]), list(L6_FACES[d])
+    up[ax] += 1
+    dn[ax] -= 1
+    return {(tuple(up), c): 0.5, (tuple(dn), c): -0.5}
+
+
+def l6_point_kind(p):
+    return ("centre", "face", "edge")[sum(abs(v) for v in p)]
+
+
+def l6_lds(p, c):
+    slot = {-1: "sm", 0: "sc", 1: "sp"}[p[2]]
+    off = "h"
+    if p[0]:
+        off += " %s 1" % ("+" if p[0] > 0 else "-")
+    if p[1]:
+        off += " %s HX" % ("+" if p[1] > 0 else "-")
+    return "q[%s][%d][%s]" % (slot, c, off)
 
 
 def gen_fast_l6(entries):
-    """L6 bulk operator reading its operands from LDS as it goes: terms grouped by
-    the first factor x_a (a Q value) and, within a group, by output component c,
-    as acc[c] += x_a * (sum_b coef x_b), one fused multiply-add per term; scheduling
-    fences between groups keep only the accumulators and a few operands live."""
-    groups = {}
+    """L6 bulk operator as a quadratic form over the 19-point values.  Every
+    product of the form pairs a face value with the centre, another face or an
+    edge value (the generator checks that), so the 30 face values are read from
+    LDS once into registers, each edge point's 5 values once as the loop over
+    edge points reaches it: 90 LDS reads per site.  Products sharing a first
+    factor x_a are summed as acc[c] += x_a * (sum_b coef x_b)."""
+    pair = {}
     for comp, coef, a, b in entries:
-        groups.setdefault(a, {}).setdefault(comp, []).append((coef, b))
+        for ka, wa in l6_point_form(a).items():
+            for kb, wb in l6_point_form(b).items():
+                key = (comp,) + tuple(sorted((ka, kb), key=lambda t: (l6_point_kind(t[0]) != "edge", t)))
+                pair[key] = pair.get(key, 0.0) + coef * wa * wb
+    pair = {k: v for k, v in pair.items() if v != 0.0}
+    blocks = {}          # None: centre/face products; edge point: its products with faces
+    for (comp, ka, kb), coef in pair.items():
+        kinds = sorted((l6_point_kind(ka[0]), l6_point_kind(kb[0])))
+        assert kinds in (["centre", "face"], ["face", "face"], ["edge", "face"]), kinds
+        edge = ka[0] if kinds[0] == "edge" else None
+        blocks.setdefault(edge, []).append((comp, ka, kb, coef))
+
+    def name(k):
+        p, c = k
+        kind = l6_point_kind(p)
+        if kind == "centre":
+            return "qc[%d]" % c
+        if kind == "face":
+            return "f%d_%d" % (L6_FACES.index(p), c)
+        return "e%d" % c
+
+    def emit_block(terms, indent):
+        # greedy grouping: the (first factor, component) pair covering the most
+        # products not yet placed becomes the next group (one multiply-add each)
+        left = list(terms)
+        out = []
+        while left:
+            count = {}
+            for comp, ka, kb, coef in left:
+                for k in (ka, kb):
+                    count[(k, comp)] = count.get((k, comp), 0) + 1
+            best = max(sorted(count), key=lambda g: count[g])
+            k, comp = best
+            group = [(coef, kb if ka == k else ka) for c2, ka, kb, coef in left if c2 == comp and k in (ka, kb)]
+            left = [t for t in left if not (t[0] == comp and k in (t[1], t[2]))]
+            out.append("%s{" % indent)
+            coef, b = group[0]
+            out.append("%s    R t = %s * %s;" % (indent, coef_literal(coef), name(b)))
+            for coef, b in group[1:]:
+                out.append("%s    t += %s * %s;" % (indent, coef_literal(coef), name(b)))
+            out.append("%s    acc[%d] += %s * t;" % (indent, comp, name(k)))
+            out.append("%s}" % indent)
+        return out
+
     lines = ["template <int HX, int HP, typename R>",
              "__device__ __forceinline__ void ldg_fast_l6(const R (*q)[5][HP], int sm, int sc, int sp, int h,"
              " const R* qc, R* acc) {"]
-    for g, a in enumerate(sorted(groups)):
-        if g and g % L6_FENCE_EVERY == 0:
-            lines.append("    __builtin_amdgcn_sched_barrier(0);")
+    for d, p in enumerate(L6_FACES):
+        for c in range(5):
+            lines.append("    const R f%d_%d = %s;" % (d, c, l6_lds(p, c)))
+    lines += emit_block(sorted(blocks[None]), "    ")
+    for p in sorted(k for k in blocks if k is not None):
+        # one edge point per block: the fence keeps the compiler from hoisting
+        # the loads of later edge points (and their registers) above it
+        lines.append("    __builtin_amdgcn_sched_barrier(0);")
         lines.append("    {")
-        lines.append("        const R xa = %s;" % l6_operand(a))
-        for comp in sorted(groups[a]):
-            terms = groups[a][comp]
-            coef, b = terms[0]
-            lines.append("        R t%d = %s * (%s);" % (comp, coef_literal(coef), l6_operand(b)))
-            for coef, b in terms[1:]:
-                lines.append("        t%d += %s * (%s);" % (comp, coef_literal(coef), l6_operand(b)))
-        for comp in sorted(groups[a]):
-            lines.append("        acc[%d] += xa * t%d;" % (comp, comp))
+        comps = sorted({t[1][1] for t in blocks[p]})
+        for c in comps:
+            lines.append("        const R e%d = %s;" % (c, l6_lds(p, c)))
+        lines += emit_block(sorted(blocks[p]), "        ")
         lines.append("    }")
     lines.append("}")
     return lines
@@ -291,9 +346,7 @@ def main():
     body = ["// Generated by open-qmin_amd/stencils/gen_stencils.py from ldg_stencil_tables.json.",
             "// Do not edit by hand.", "#pragma once", ""]
     body += gen_bulk_linear(tables)
-    body += ["", "template <typename R>", "__device__ __forceinline__ void ldg_bulk_l6(const R* x, R* out) {"]
-    gen_l6(tables["L6"]["bulk"], body)
-    body += ["}", ""]
+    body += [""]
     body += gen_boundary(tables, cases)
     fast_dev, fast_host = gen_fast(tables)
     body += [""] + fast_dev
