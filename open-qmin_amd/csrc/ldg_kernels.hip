@@ -108,12 +108,11 @@ __device__ __forceinline__ void fast_raw_force(const ForceConst& fc, ConstPtr<R>
         l1_shape(spatial);
     } else if constexpr (TERMS == kMultiLinear) {
         ldg_fast_lin_a<HX, HP>(W, q, sm, sc, sp, h, spatial);
-    } else {
+    } else if constexpr (TERMS == kMultiL4) {
         ldg_fast_lin_b<HX, HP>(W, q, sm, sc, sp, h, spatial);
-    }
-    if constexpr (TERMS == kMultiL6) {
+    } else {
         R o6[5] = {R(0), R(0), R(0), R(0), R(0)};
-        ldg_fast_l6<HX, HP>(q, sm, sc, sp, h, qc, o6);
+        ldg_fast_lin_b_l6<HX, HP>(W, q, sm, sc, sp, h, qc, spatial, o6);
 #pragma unroll
         for (int c = 0; c < 5; ++c) spatial[c] += R(fc.L6) * o6[c];
     }

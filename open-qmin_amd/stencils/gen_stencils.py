@@ -27,9 +27,11 @@ differences into one linear operator on the 19-point neighbourhood (centre,
   ../csrc/gen/ldg_fast_host.inc                   host: C1..C4 per entry k
 variant a covers L1..L3 (L4 = 0), variant b L1..L4.  The bulk L6 form, a
 quadratic form in the same central differences, becomes one over the 19 point
-values:
+values, generated together with variant b so that each value is read once:
 
-  ldg_fast_l6(q, sm, sc, sp, h, qc, acc)          device: acc[c] += x_a (sum_b coef x_b)
+  ldg_fast_lin_b_l6(W, q, sm, sc, sp, h, qc, acc, acc6)
+                                      device: variant b into acc, and
+                                      acc6[c] += x_a (sum_b coef x_b)
 """
 import json
 import os
@@ -227,7 +229,7 @@ def gen_fast(tables):
                 last = (p, c)
             dev.append("    acc[%d] += W[%d] * v;" % (r, idx))
         dev.append("}")
-    return dev, host
+    return dev, host, keys
 
 
 L6_FACES = ((-1, 0, 0), (1, 0, 0), (0, -1, 0), (0, 1, 0), (0, 0, -1), (0, 0, 1))
@@ -265,13 +267,15 @@ def l6_lds(p, c):
     return "q[%s][%d][%s]" % (slot, c, off)
 
 
-def gen_fast_l6(entries):
-    """L6 bulk operator as a quadratic form over the 19-point values.  Every
-    product of the form pairs a face value with the centre, another face or an
-    edge value (the generator checks that), so the 30 face values are read from
-    LDS once into registers, each edge point's 5 values once as the loop over
-    edge points reaches it: 90 LDS reads per site.  Products sharing a first
-    factor x_a are summed as acc[c] += x_a * (sum_b coef x_b)."""
+def gen_fast_l6(entries, lin_keys):
+    """Bulk L1..L4 operator (variant b, weights W) and L6 form in one pass over
+    the 19 point values.  The L6 form is a quadratic form over those values;
+    every product pairs a face value with the centre, another face or an edge
+    value (the generator checks that), so the 30 face values are read from LDS
+    once into registers and each edge point's 5 values once as the loop over
+    edge points reaches it, and each value read also feeds its linear terms:
+    90 LDS reads per site for both forms.  L6 products sharing a first factor
+    x_a are summed as acc6[c] += x_a * (sum_b coef x_b)."""
     pair = {}
     for comp, coef, a, b in entries:
         for ka, wa in l6_point_form(a).items():
@@ -314,27 +318,39 @@ def gen_fast_l6(entries):
             out.append("%s    R t = %s * %s;" % (indent, coef_literal(coef), name(b)))
             for coef, b in group[1:]:
                 out.append("%s    t += %s * %s;" % (indent, coef_literal(coef), name(b)))
-            out.append("%s    acc[%d] += %s * t;" % (indent, comp, name(k)))
+            out.append("%s    acc6[%d] += %s * t;" % (indent, comp, name(k)))
             out.append("%s}" % indent)
         return out
 
-    lines = ["template <int HX, int HP, typename R>",
-             "__device__ __forceinline__ void ldg_fast_l6(const R (*q)[5][HP], int sm, int sc, int sp, int h,"
-             " const R* qc, R* acc) {"]
+    lin_at = {}
+    for idx, (r, p, c) in enumerate(lin_keys):
+        lin_at.setdefault((p, c), []).append((idx, r))
+
+    def emit_lin(p, c, value, indent):
+        return ["%sacc[%d] += W[%d] * %s;" % (indent, r, idx, value) for idx, r in lin_at.pop((p, c), [])]
+
+    lines = ["template <int HX, int HP, typename R, typename WP>",
+             "__device__ __forceinline__ void ldg_fast_lin_b_l6(WP W, const R (*q)[5][HP], int sm, int sc, int sp,"
+             " int h, const R* qc, R* acc, R* acc6) {"]
+    for c in range(5):
+        lines += emit_lin((0, 0, 0), c, "qc[%d]" % c, "    ")
     for d, p in enumerate(L6_FACES):
         for c in range(5):
             lines.append("    const R f%d_%d = %s;" % (d, c, l6_lds(p, c)))
+            lines += emit_lin(p, c, "f%d_%d" % (d, c), "    ")
     lines += emit_block(sorted(blocks[None]), "    ")
-    for p in sorted(k for k in blocks if k is not None):
+    edge_pts = sorted({p for (p, c) in lin_at} | {k for k in blocks if k is not None})
+    for p in edge_pts:
         # one edge point per block: the fence keeps the compiler from hoisting
         # the loads of later edge points (and their registers) above it
         lines.append("    __builtin_amdgcn_sched_barrier(0);")
         lines.append("    {")
-        comps = sorted({t[1][1] for t in blocks[p]})
-        for c in comps:
+        for c in range(5):
             lines.append("        const R e%d = %s;" % (c, l6_lds(p, c)))
-        lines += emit_block(sorted(blocks[p]), "        ")
+            lines += emit_lin(p, c, "e%d" % c, "        ")
+        lines += emit_block(sorted(blocks.get(p, [])), "        ")
         lines.append("    }")
+    assert not lin_at, "linear terms outside the 19-point neighbourhood"
     lines.append("}")
     return lines
 
@@ -348,9 +364,9 @@ def main():
     body += gen_bulk_linear(tables)
     body += [""]
     body += gen_boundary(tables, cases)
-    fast_dev, fast_host = gen_fast(tables)
+    fast_dev, fast_host, lin_keys = gen_fast(tables)
     body += [""] + fast_dev
-    body += [""] + gen_fast_l6(tables["L6"]["bulk"])
+    body += [""] + gen_fast_l6(tables["L6"]["bulk"], lin_keys)
     host_text = "\n".join(fast_host) + "\n"
     host_out = os.path.join(os.path.dirname(OUT), "ldg_fast_host.inc")
     if not os.path.exists(host_out) or open(host_out).read() != host_text:
