@@ -176,7 +176,7 @@ def ensemble_throughput(qmin, count=1024, fc=1e-7, max_iter=10000, sequential_sa
     return {"lattices": count, "sites": int(sum(sites)), "dtype": "f32 (FP64 reductions)",
             "forceMax_below": fc, "max_iterations": max_iter, "seconds": t2 - t1, "build_seconds": build_s,
             "site_updates_per_s": updates / (t2 - t1), "converged": int(sum(1 for f in fms if f <= fc)),
-            "autotune": {"tile": ["32x8", "64x4", "32x16"][tile], "zchunk": zchunk, "pass_ms": pass_ms,
+            "autotune": {"tile": ["32x8", "64x4", "32x16", "64x8"][tile], "zchunk": zchunk, "pass_ms": pass_ms,
                          "seconds": t1 - ta},
             "iterations_min_mean_max": [int(min(its)), float(sum(its)) / count, int(max(its))],
             "one_by_one": {"lattices": sequential_sample, "site_updates_per_s": seq_updates / (t4 - t3),
@@ -347,7 +347,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": measured_traffic(args.size, args.precision),
                      "kernel": "fused FIRE lattice pass (%s)" % (
-                         "march_wide_kernel, 32x16 tile" if args.size >= 128 else "march_kernel, 32x8 / 64x4 tile"),
+                         ("march_wide_kernel, %s tile" % ("32x16" if precision == qmin.QM_FP64 else "64x8"))
+                         if args.size >= 128 else "march_kernel, 32x8 / 64x4 tile"),
                      "kernel_ms": r["march_ms"], "boundary_kernel_ms": r["boundary_ms"], "bytes_per_site": bps},
         "iteration_hbm_frac": value * bps / (PEAK_HBM_GBS * 1e9),
         "forceMax_after": r["forceMax"],

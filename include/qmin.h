@@ -62,7 +62,7 @@ int qm_lattice_dims(const qm_lattice* lat, int* lx, int* ly, int* lz, int* preci
  * knob of this implementation; it has no counterpart in the reference. */
 int qm_lattice_set_zchunk(qm_lattice* lat, int zchunk);
 /* tile shape of the fused lattice pass: -1 (default) picks by precision, 0 is
- * 32x8 sites per work-group, 1 is 64x4, 2 is 32x16.  A tuning knob of this
+ * 32x8 sites per work-group, 1 is 64x4, 2 is 32x16, 3 is 64x8.  A tuning knob of this
  * implementation. */
 int qm_lattice_set_tile(qm_lattice* lat, int tile);
 /* tile autotuner, the role of kernelTuner (src/utilities/kernelTuner.cpp:5-125):

@@ -341,11 +341,12 @@ __global__ void __launch_bounds__(kTileThreads[TILE]) march_kernel(MarchArgs<R> 
     march_body<R, MODE, TERMS, TILE>(A, xcd_logical(blockIdx.x, A.nblocks), blockIdx.x);
 }
 
-// The 32 x 16 tile (512 threads, 8 waves): two work-groups per CU need at least
-// four waves per SIMD, i.e. at most 128 VGPRs.
-template <typename R, int MODE, int TERMS>
-__global__ void __launch_bounds__(kTileThreads[kTile32x16], 4) march_wide_kernel(MarchArgs<R> A) {
-    march_body<R, MODE, TERMS, kTile32x16>(A, xcd_logical(blockIdx.x, A.nblocks), blockIdx.x);
+// The 512-thread tiles (32 x 16, 64 x 8; 8 waves): two work-groups per CU need
+// at least four waves per SIMD, i.e. at most 128 VGPRs.
+template <typename R, int MODE, int TERMS, int TILE>
+__global__ void __launch_bounds__(kTileThreads[TILE], 4) march_wide_kernel(MarchArgs<R> A) {
+    static_assert(kTileThreads[TILE] == 512, "512-thread tiles only");
+    march_body<R, MODE, TERMS, TILE>(A, xcd_logical(blockIdx.x, A.nblocks), blockIdx.x);
 }
 
 // The fused pass over many lattices in one launch: map[b] = {lattice, logical
@@ -359,12 +360,13 @@ __global__ void __launch_bounds__(kTileThreads[TILE]) march_batch_kernel(const M
     march_body<R, MODE, TERMS, TILE>(descs[lat], logical, logical);
 }
 
-template <typename R, int MODE, int TERMS>
-__global__ void __launch_bounds__(kTileThreads[kTile32x16], 4) march_wide_batch_kernel(
+template <typename R, int MODE, int TERMS, int TILE>
+__global__ void __launch_bounds__(kTileThreads[TILE], 4) march_wide_batch_kernel(
     const MarchArgs<R>* __restrict__ descs, const int2* __restrict__ map) {
+    static_assert(kTileThreads[TILE] == 512, "512-thread tiles only");
     const int lat = __builtin_amdgcn_readfirstlane(map[blockIdx.x].x);
     const int logical = __builtin_amdgcn_readfirstlane(map[blockIdx.x].y);
-    march_body<R, MODE, TERMS, kTile32x16>(descs[lat], logical, logical);
+    march_body<R, MODE, TERMS, TILE>(descs[lat], logical, logical);
 }
 
 // ---------------------------------------------------------------- boundary pass
@@ -873,7 +875,8 @@ template <typename R, int MODE, int TERMS>
 static const void* march_fn_tile(int tile) {
     switch (tile) {
         case kTile64x4: return reinterpret_cast<const void*>(&march_kernel<R, MODE, TERMS, kTile64x4>);
-        case kTile32x16: return reinterpret_cast<const void*>(&march_wide_kernel<R, MODE, TERMS>);
+        case kTile32x16: return reinterpret_cast<const void*>(&march_wide_kernel<R, MODE, TERMS, kTile32x16>);
+        case kTile64x8: return reinterpret_cast<const void*>(&march_wide_kernel<R, MODE, TERMS, kTile64x8>);
         default: return reinterpret_cast<const void*>(&march_kernel<R, MODE, TERMS, kTile32x8>);
     }
 }
@@ -902,7 +905,8 @@ template <typename R, int MODE, int TERMS>
 static const void* march_batch_fn_tile(int tile) {
     switch (tile) {
         case kTile64x4: return reinterpret_cast<const void*>(&march_batch_kernel<R, MODE, TERMS, kTile64x4>);
-        case kTile32x16: return reinterpret_cast<const void*>(&march_wide_batch_kernel<R, MODE, TERMS>);
+        case kTile32x16: return reinterpret_cast<const void*>(&march_wide_batch_kernel<R, MODE, TERMS, kTile32x16>);
+        case kTile64x8: return reinterpret_cast<const void*>(&march_wide_batch_kernel<R, MODE, TERMS, kTile64x8>);
         default: return reinterpret_cast<const void*>(&march_batch_kernel<R, MODE, TERMS, kTile32x8>);
     }
 }

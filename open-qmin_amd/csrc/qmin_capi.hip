@@ -331,7 +331,7 @@ int qm_lattice_load_hfield(qm_lattice* L, const char* path) {
 }
 
 int qm_lattice_set_tile(qm_lattice* L, int tile) {
-    if (!L || tile < -1 || tile >= kNumTiles) return fail(QM_EINVAL, "tile is -1 (automatic), 0 (32x8), 1 (64x4) or 2 (32x16)");
+    if (!L || tile < -1 || tile >= kNumTiles) return fail(QM_EINVAL, "tile is -1 (automatic), 0 (32x8), 1 (64x4), 2 (32x16) or 3 (64x8)");
     L->tileOverride = tile;
     return QM_OK;
 }
@@ -842,13 +842,15 @@ static int upload_weights(qm_lattice* L, const qm_ldg* f) {
 }
 
 // Tile shape of the fused pass: the one set on the lattice, else the measured
-// better default: 32 x 16 on lattices at least 128 sites wide in x and y (256^3:
-// 2% faster than 32 x 8 in FP64, 6% faster than 64 x 4 in FP32); on smaller ones,
-// and with the L6 term (whose pass needs more than the 128 registers the 32 x 16
-// tile allows), 32 x 8 for FP64 and 64 x 4 for FP32.
+// better default.  On lattices at least 128 sites wide in x and y: 32 x 16 in
+// FP64 (256^3: 2% faster than 32 x 8, 3% faster than 64 x 8) and 64 x 8 in FP32
+// (6% faster than 32 x 16, 11% faster than 64 x 4).  On smaller ones, and with
+// the L6 term (whose pass needs more than the 128 registers the 512-thread tiles
+// allow), 32 x 8 for FP64 and 64 x 4 for FP32.
 int qmin::active_tile(const qm_lattice* L) {
     if (L->tileOverride >= 0) return L->tileOverride;
-    if (L->terms != kMultiL6 && L->lx >= 128 && L->ly >= 128) return kTile32x16;
+    if (L->terms != kMultiL6 && L->lx >= 128 && L->ly >= 128)
+        return L->precision == QM_FP64 ? kTile32x16 : kTile64x8;
     return L->precision == QM_FP64 ? kTile32x8 : kTile64x4;
 }
 

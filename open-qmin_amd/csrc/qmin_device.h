@@ -7,14 +7,15 @@ namespace qmin {
 
 // Work-group of the fused pass: one thread per site of a TX x TY tile with a
 // one-site halo in x and y; the work-group marches along z through a chunk.
-// Three tile shapes are compiled: 32 x 8 and 64 x 4 (256 threads; wider rows use
-// whole cache lines for the side halo but have more halo sites per tile) and
-// 32 x 16 (512 threads, 73 KB of LDS in FP64: a fifth fewer halo sites per site).
+// Four tile shapes are compiled: 32 x 8 and 64 x 4 (256 threads; wider rows use
+// whole cache lines for the side halo but have more halo sites per tile), and
+// 32 x 16 and 64 x 8 (512 threads, 73-79 KB of LDS in FP64: fewer halo sites per
+// site; with 64 x 8 each wave covers one contiguous 64-site row).
 constexpr int kThreads = 256;   // work-group size of the streaming kernels
-enum TileId { kTile32x8 = 0, kTile64x4 = 1, kTile32x16 = 2, kNumTiles = 3 };
-constexpr int kTileTX[kNumTiles] = {32, 64, 32};
-constexpr int kTileTY[kNumTiles] = {8, 4, 16};
-constexpr int kTileThreads[kNumTiles] = {32 * 8, 64 * 4, 32 * 16};
+enum TileId { kTile32x8 = 0, kTile64x4 = 1, kTile32x16 = 2, kTile64x8 = 3, kNumTiles = 4 };
+constexpr int kTileTX[kNumTiles] = {32, 64, 32, 64};
+constexpr int kTileTY[kNumTiles] = {8, 4, 16, 8};
+constexpr int kTileThreads[kNumTiles] = {32 * 8, 64 * 4, 32 * 16, 64 * 8};
 
 template <int TX_, int TY_>
 struct Tile {

@@ -206,7 +206,7 @@ class Lattice:
         _check(self.lib.qm_lattice_load_hfield(self.h, str(path).encode()))
 
     def set_tile(self, tile):
-        """Tile shape of the fused pass: -1 automatic, 0 = 32x8, 1 = 64x4, 2 = 32x16."""
+        """Tile shape of the fused pass: -1 automatic, 0 = 32x8, 1 = 64x4, 2 = 32x16, 3 = 64x8."""
         _check(self.lib.qm_lattice_set_tile(self.h, tile))
 
     def autotune(self, force):

@@ -87,7 +87,7 @@ def test_ensemble_autotune_leaves_states_and_results_unchanged():
     q0 = [l.q() for l in lats]
     ens = qmin.Ensemble(lats)
     tile, zchunk, ms = ens.autotune(force)
-    assert tile in (0, 1, 2) and zchunk in (8, 16, 32, 64) and ms > 0
+    assert tile in (0, 1, 2, 3) and zchunk in (8, 16, 32, 64) and ms > 0
     for l, q in zip(lats, q0):
         # equal_nan: the object site at an integer sphere centre has no direction
         # (0/0 in the reference's Q^B as well); it never enters a force
