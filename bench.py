@@ -110,27 +110,37 @@ def time_to_tolerance(qmin, L, mode, fc=1e-8, switch_fc=None, max_iter=400000):
 
 
 def warm_start(qmin, L, fc=1e-8):
-    """FP64 FIRE to forceMax < fc on the bench workload from the trained
-    network's prediction (committed weights) and from a uniform director along
-    the same far-field direction (x, in the plane of the planar walls); time
-    includes the prediction."""
+    """FIRE to forceMax < fc on the bench workload from the trained network's
+    prediction (committed weights) and from a uniform director along the same
+    far-field direction (x, in the plane of the planar walls), in FP64; and from
+    the prediction in mixed precision (FP32 to fc, then the FP64 polish as in
+    time_to_tolerance).  Times include the prediction."""
     from qmin import warmstart as ws
     path = os.path.join(REPO, "open-qmin_amd", "warmstart", "weights_64.npz")
     net = ws.WarmStartNet(seed=0)
     net.load(path)
     A, B, C, S0 = phase_constants()
     out = {}
-    for mode in ("network", "uniform_director"):
-        lat, force = build_workload(qmin, L, qmin.QM_FP64, uniform=(1.0, 0.0, 0.0) if mode != "network" else None)
+    for mode in ("network", "uniform_director", "network_mixed"):
+        mixed = mode == "network_mixed"
+        lat, force = build_workload(qmin, L, qmin.QM_FP32 if mixed else qmin.QM_FP64,
+                                    uniform=None if mode.startswith("network") else (1.0, 0.0, 0.0))
         m = qmin.FIRE(fc=fc, max_iter=400000, **fire_params())
         t0 = time.perf_counter()
-        if mode == "network":
+        if mode.startswith("network"):
             net.predict(lat, (1.0, 0.0, 0.0), S0)
         st = m.minimize(force, lat)
+        first = st.iterations
+        if mixed:
+            lat.set_precision(qmin.QM_FP64)
+            m.set_max_iterations(first + 400000)
+            st = m.minimize(force, lat)
         t1 = time.perf_counter()
         e, _ = force.compute_energy(lat)
         out[mode] = dict(seconds=t1 - t0, iterations=st.iterations, forceMax=st.forceMax,
                          converged=bool(st.converged), energy_per_site=e / lat.n_active())
+        if mixed:
+            out[mode].update(fp32_iterations=first, polish_iterations=st.iterations - first)
         print("time to forceMax<%g (%s start): %d iterations, %.2f s" % (fc, mode, st.iterations, t1 - t0),
               file=sys.stderr, flush=True)
     return out
