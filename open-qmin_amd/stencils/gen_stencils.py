@@ -59,11 +59,6 @@ def split_input(i):
     return ("d", j // 15, (j % 15) // 5, j % 5)
 
 
-def emit_sum(lines, target, terms, var):
-    for coef, name in terms:
-        lines.append("    %s += %s * %s;" % (target, coef_literal(coef), name))
-
-
 def gen_bulk_linear(tables):
     out = []
     # centre
