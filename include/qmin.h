@@ -180,6 +180,11 @@ int qm_ldg_compute_forces(qm_ldg* f, qm_lattice* lat);
  * the reference (QM_EINVAL otherwise). */
 int qm_ldg_object_force(qm_ldg* f, qm_lattice* lat, int object, double* force);
 int qm_ldg_compute_energy(qm_ldg* f, qm_lattice* lat, double* total, double* components5);
+/* landauDeGennesLC::energyDensity (landauDeGennesLC.h:96) as computeEnergy fills
+ * it (landauDeGennesLC.cpp:204-353): density[n] in site order, 0 on object sites,
+ * else the site's phase, field, anchoring and distortion energy; the values that
+ * qm_ldg_compute_energy sums */
+int qm_ldg_energy_density(qm_ldg* f, qm_lattice* lat, double* density);
 
 /* ---- FIRE: energyMinimizerFIRE (src/updaters/energyMinimizerFIRE.h:20-110) */
 typedef struct {

@@ -754,6 +754,26 @@ __global__ void __launch_bounds__(kReduceThreads) energy_kernel(const R* q, cons
     }
 }
 
+// Per-site energy density (landauDeGennesLC::energyDensity): 0 on object
+// sites, else phase, fields, anchoring and distortion, added in the order of
+// computeEnergyCPU (landauDeGennesLC.cpp:204-353).
+template <typename R>
+__global__ void __launch_bounds__(kReduceThreads) energy_density_kernel(const R* q, const int8_t* types,
+                                                                        const DevObject* objects, int lx, int ly,
+                                                                        int lz, EnergyConst ec, double* out) {
+    const long long n = (long long)lx * ly * lz;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        if (types[i] > 0) {
+            out[i] = 0.0;
+            continue;
+        }
+        double parts[5];
+        R dv[3][5];
+        site_energy(q, types, objects, lx, ly, lz, ec, i, parts, dv);
+        out[i] = parts[0] + parts[3] + parts[4] + parts[2] + parts[1];
+    }
+}
+
 // Force on an object from its surface sites (computeObjectForces with
 // computeStressTensors, landauDeGennesLCOtherForces.cpp:147-251; one-constant):
 // per surface site, the stress tensor of the first derivatives and L1 with the
@@ -1029,6 +1049,16 @@ int launch_energy(const R* q, const int8_t* types, const DevObject* objects, int
 }
 
 template <typename R>
+void launch_energy_density(const R* q, const int8_t* types, const DevObject* objects, int lx, int ly, int lz,
+                           const EnergyConst& ec, double* out, hipStream_t s) {
+    const long long n = (long long)lx * ly * lz;
+    long long blocks = (n + kReduceThreads - 1) / kReduceThreads;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL((energy_density_kernel<R>), dim3(blocks), dim3(kReduceThreads), 0, s, q, types, objects, lx,
+                       ly, lz, ec, out);
+}
+
+template <typename R>
 void launch_defects(const R* q, const int8_t* types, long long n, int kind, double* out, hipStream_t s) {
     long long blocks = (n + 255) / 256;
     if (blocks > 65536) blocks = 65536;
@@ -1042,6 +1072,8 @@ void launch_defects(const R* q, const int8_t* types, long long n, int kind, doub
     template int launch_energy<R>(const R*, const int8_t*, const DevObject*, int, int, int, const EnergyConst&,   \
                                   double*, hipStream_t);                                                        \
     template void launch_defects<R>(const R*, const int8_t*, long long, int, double*, hipStream_t);             \
+    template void launch_energy_density<R>(const R*, const int8_t*, const DevObject*, int, int, int,              \
+                                           const EnergyConst&, double*, hipStream_t);                            \
     template int march_blocks_per_cu<R>(int, int, int);                                                          \
     template hipError_t launch_march_batch<R>(const MarchArgs<R>*, const int2*, int, int, int, int, hipStream_t);  \
     template hipError_t launch_boundary_batch<R>(const BoundaryArgs<R>*, const int2*, int, int, int, hipStream_t);  \

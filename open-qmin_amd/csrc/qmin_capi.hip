@@ -36,6 +36,9 @@ int launch_energy(const R* q, const int8_t* types, const DevObject* objects, int
 template <typename R>
 void launch_defects(const R* q, const int8_t* types, long long n, int kind, double* out, hipStream_t s);
 template <typename R>
+void launch_energy_density(const R* q, const int8_t* types, const DevObject* objects, int lx, int ly, int lz,
+                           const EnergyConst& ec, double* out, hipStream_t s);
+template <typename R>
 void launch_object_force(const R* q, const int8_t* types, const DevObject* objects, int lx, int ly, int lz,
                          const EnergyConst& ec, double L1, const int64_t* sites, long long m, double* out,
                          hipStream_t s);
@@ -1172,6 +1175,27 @@ int qm_ldg_compute_energy(qm_ldg* f, qm_lattice* L, double* total, double* comps
     if (comps)
         for (int k = 0; k < 5; ++k) comps[k] = c5[k];
     if (total) *total = c5[0] + c5[1] + c5[2] + c5[3] + c5[4];
+    return QM_OK;
+}
+
+int qm_ldg_energy_density(qm_ldg* f, qm_lattice* L, double* density) {
+    if (!density) return fail(QM_EINVAL, "null output");
+    if (int rc = prepare(L, f)) return rc;
+    const EnergyConst ec = energy_const(f, L);
+    double* dOut = nullptr;
+    HIP_TRY(hipMalloc(&dOut, sizeof(double) * (size_t)L->n));
+    if (L->precision == QM_FP64)
+        launch_energy_density<double>(state_set<double>(L, L->hCtrl.cur).q, L->dTypes, L->dObjects, L->lx, L->ly,
+                                      L->lz, ec, dOut, L->stream);
+    else
+        launch_energy_density<float>(state_set<float>(L, L->hCtrl.cur).q, L->dTypes, L->dObjects, L->lx, L->ly,
+                                     L->lz, ec, dOut, L->stream);
+    hipError_t err = hipGetLastError();
+    if (err == hipSuccess)
+        err = hipMemcpyAsync(density, dOut, sizeof(double) * (size_t)L->n, hipMemcpyDeviceToHost, L->stream);
+    if (err == hipSuccess) err = hipStreamSynchronize(L->stream);
+    (void)hipFree(dOut);
+    if (err != hipSuccess) return fail(QM_EHIP, std::string("energy density: ") + hipGetErrorString(err));
     return QM_OK;
 }
 

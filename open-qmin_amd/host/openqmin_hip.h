@@ -318,11 +318,14 @@ class landauDeGennesLC {
         qmin_host::check(qm_ldg_compute_forces(f_, lattice()), "computeForces");
     }
     // landauDeGennesLC::computeEnergy: the total (not per site); fills
-    // energyComponents {phase, distortion, anchoring, E, H}
+    // energyComponents {phase, distortion, anchoring, E, H} and, as the
+    // reference does, the per-site energyDensity
     scalar computeEnergy(bool verbose = false) {
         double total = 0;
         energyComponents.assign(5, 0.0);
         qmin_host::check(qm_ldg_compute_energy(f_, lattice(), &total, energyComponents.data()), "computeEnergy");
+        energyDensity.assign((size_t)model->getNumberOfParticles(), 0.0);
+        qmin_host::check(qm_ldg_energy_density(f_, lattice(), energyDensity.data()), "computeEnergy");
         if (verbose)
             std::printf("energy: phase %g distortion %g anchoring %g E %g H %g\n", energyComponents[0],
                         energyComponents[1], energyComponents[2], energyComponents[3], energyComponents[4]);
@@ -348,6 +351,8 @@ class landauDeGennesLC {
     }
 
     std::vector<scalar> energyComponents;
+    //! the free energy density at each lattice site (landauDeGennesLC.h:96), site order
+    std::vector<scalar> energyDensity;
     scalar A = -1, B = 0, C = 0, L1 = 2.32, L2 = 0, L3 = 0, L4 = 0, L6 = 0;
     distortionEnergyType numberOfConstants = distortionEnergyType::oneConstant;
 

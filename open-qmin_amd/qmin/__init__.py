@@ -95,6 +95,7 @@ _API = {
     "qm_ldg_set_semantics": [_VP, _i],
     "qm_ldg_compute_forces": [_VP, _VP],
     "qm_ldg_compute_energy": [_VP, _VP, _D, _D],
+    "qm_ldg_energy_density": [_VP, _VP, _D],
     "qm_fire_create": [ctypes.POINTER(_VP)],
     "qm_fire_destroy": [_VP],
     "qm_fire_set_parameters": [_VP, _d, _d, _d, _d, _d, _d, _i, _d, _d],
@@ -390,6 +391,13 @@ class LandauDeGennesLC:
         comps = np.zeros(5)
         _check(self.lib.qm_ldg_compute_energy(self.h, lattice.h, ctypes.byref(tot), _dp(comps)))
         return tot.value, comps
+
+    def energy_density(self, lattice):
+        """landauDeGennesLC::energyDensity: per-site energy, site order (0 on
+        object sites); compute_energy's total is its sum."""
+        out = np.zeros(lattice.n)
+        _check(self.lib.qm_ldg_energy_density(self.h, lattice.h, _dp(out)))
+        return out
 
     def object_force(self, lattice, obj):
         """computeObjectForces: the stress-tensor force on object obj (0-based);

@@ -88,6 +88,7 @@ def oracle_lib():
         _sig(lib, "orc_forces", None, [_VP, _D, _D])
         _sig(lib, "orc_first_derivatives", None, [_VP, _D, _D])
         _sig(lib, "orc_energy", _d, [_VP, _D, _D])
+        _sig(lib, "orc_energy_density", None, [_VP, _D, _D])
         _sig(lib, "orc_fire", _i, [_VP, _d, _d, _d, _d, _d, _d, _i, _d, _d, _i, _D])
         _sig(lib, "orc_adam", _i, [_VP, _d, _d, _d, _d, _d, _i, _D])
         _sig(lib, "orc_nesterov", _i, [_VP, _d, _d, _d, _i, _i, _D])
@@ -297,6 +298,13 @@ class OracleSystem:
         comps = np.zeros(5)
         total = self.lib.orc_energy(self.h, dptr(q), dptr(comps))
         return total, comps
+
+    def energy_density(self, q=None):
+        """Per-site energy density (landauDeGennesLC::energyDensity), site order."""
+        q = self.q() if q is None else np.ascontiguousarray(q, np.float64)
+        out = np.zeros(self.n)
+        self.lib.orc_energy_density(self.h, dptr(q), dptr(out))
+        return out
 
     def fire(self, dt=5e-4, alpha_start=0.99, dt_max=None, dt_inc=1.1, dt_dec=0.95, alpha_dec=0.9, n_min=4,
              fc=1e-12, alpha_min=0.0, max_iter=100):

@@ -850,6 +850,48 @@ double orc_energy(orc_system* s, const double* q, double* comps) {
     return phase + distortion + anchoring + eF + hF;
 }
 
+// landauDeGennesLC::energyDensity (landauDeGennesLC.h:96) as computeEnergyCPU
+// fills it (landauDeGennesLC.cpp:204-353): per site, 0 for object sites, else
+// phase, then E, uniform H and spatially varying H, anchoring and distortion,
+// added in that order.
+void orc_energy_density(orc_system* s, const double* q, double* out) {
+    const double a = 0.5 * s->A, b = s->B / 3.0, c = 0.25 * s->C;
+    const double Ls[5] = {s->L1, s->L2, s->L3, s->L4, s->L6};
+    for (int64_t i = 0; i < s->n; ++i) {
+        out[i] = 0.0;
+        const int t = s->types[i];
+        if (t > 0) continue;
+        const double* qc = q + 5 * i;
+        const double t2 = trQ2(qc);
+        out[i] += a * t2 + b * trQ3(qc) + c * t2 * t2;
+        if (s->eOn) out[i] += fieldEnergy(s->E, s->eps, s->deps, s->eps0, qc);
+        if (s->hOn) out[i] += fieldEnergy(s->H, s->chi, s->dchi, s->mu0, qc);
+        if (s->hvarOn) out[i] += fieldEnergy(&s->hvar[3 * i], s->chi, s->dchi, s->mu0, qc);
+        const int32_t* nb = &s->nbr[6 * i];
+        double d[3][5];
+        for (int ax = 0; ax < 3; ++ax)
+            for (int k = 0; k < 5; ++k) d[ax][k] = 0.5 * (q[5 * (int64_t)nb[2 * ax + 1] + k] - q[5 * (int64_t)nb[2 * ax] + k]);
+        if (t < 0) {
+            double anch = 0;
+            for (int dd = 0; dd < 6; ++dd) {
+                const int tn = s->types[nb[dd]];
+                if (tn <= 0) continue;
+                anch += anchoringEnergy(s->objects[tn - 1], qc, q + 5 * (int64_t)nb[dd]);
+                const int ax = dd / 2;
+                const double* other = q + 5 * (int64_t)nb[dd ^ 1];
+                for (int k = 0; k < 5; ++k) d[ax][k] = (dd & 1) ? (qc[k] - other[k]) : (other[k] - qc[k]);
+            }
+            out[i] += anch;
+        }
+        double e[5];
+        distortionTerms(qc, d[0], d[1], d[2], e);
+        double dist = 0;
+        for (int k = 0; k < 5; ++k)
+            if (Ls[k] != 0) dist += Ls[k] * e[k];
+        out[i] += dist;
+    }
+}
+
 // landauDeGennesLC::computeObjectForces (landauDeGennesLCOtherForces.cpp:147-208)
 // with computeStressTensors (:214-251; one-constant only, as there): over the
 // object's surface sites, the stress tensor from the first derivatives

@@ -62,6 +62,8 @@ void orc_forces(orc_system* s, const double* q, double* f);
 void orc_first_derivatives(orc_system* s, const double* q, double* d15);
 // returns the total energy; comps = {phase, distortion, anchoring, E, H}
 double orc_energy(orc_system* s, const double* q, double* comps);
+// per-site energy density (landauDeGennesLC::energyDensity), n values; 0 on object sites
+void orc_energy_density(orc_system* s, const double* q, double* out);
 // force callback usable by the reference harness (ctx = orc_system*)
 void orc_force_cb(void* ctx, int n, const double* q, double* f);
 double orc_energy_cb(void* ctx, int n, const double* q);

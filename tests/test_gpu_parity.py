@@ -97,6 +97,28 @@ def test_energy_fp64(params):
     np.testing.assert_allclose(c_got, c_ref, rtol=1e-11, atol=1e-12 * abs(e_ref))
 
 
+@pytest.mark.parametrize("params", [ONE, MULTI, ALL])
+def test_energy_density_fp64(params):
+    """landauDeGennesLC::energyDensity: every site's energy against the oracle's
+    (pinned there by its sum and locality), zero on object sites, summing to
+    compute_energy's total."""
+    o, g, f = build_pair((24, 18, 14), params, 0, True, efield=(0.0, 0.1, 0.4))
+    want = o.energy_density()
+    got = f.energy_density(g)
+    assert got.shape == want.shape
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-12 * np.abs(want).max())
+    assert np.all(got[o.types() > 0] == 0.0)
+    total, _ = f.compute_energy(g)
+    np.testing.assert_allclose(got.sum(), total, rtol=1e-12)
+
+
+def test_energy_density_fp32_close():
+    o, g, f = build_pair((24, 18, 14), ALL, 0, True, efield=(0.0, 0.1, 0.4), precision=qmin.QM_FP32)
+    want = o.energy_density()
+    got = f.energy_density(g)
+    np.testing.assert_allclose(got, want, rtol=0, atol=2e-5 * np.abs(want).max())
+
+
 def test_defects():
     o, g, _ = build_pair((16, 12, 10), MULTI, 0, True)
     lc = o.types() <= 0

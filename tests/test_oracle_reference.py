@@ -399,3 +399,57 @@ def test_object_force_properties(phase):
     o.fire(max_iter=60)
     f = o.object_force(1)
     assert np.abs(f[:2]).max() < 1e-9 * abs(f[2]) + 1e-13 and abs(f[2]) > 1e-6
+
+
+def _q_matrix(q5):
+    """Traceless symmetric Q from (Qxx, Qxy, Qxz, Qyy, Qyz)."""
+    a, b, c, d, e = q5
+    return np.array([[a, b, c], [b, d, e], [c, e, -a - d]])
+
+
+def test_energy_density_sums_to_total_and_is_local():
+    """The oracle's per-site energy density (landauDeGennesLC::energyDensity,
+    filled by computeEnergyCPU, landauDeGennesLC.cpp:204-353): it sums to the
+    total (pinned above against the reference's energy polynomials), is zero on
+    object sites, equals a TrQ^2 + b TrQ^3 + c (TrQ^2)^2 computed here from the
+    3x3 matrix on a uniform state (no gradients, no fields), and a change of Q at
+    one site changes only that site's density and its six neighbours'."""
+    A, B, C = po.rescaled_phase()
+    S0 = po.uniaxial_s0(A, B, C)
+    shape = (9, 8, 7)
+    o = po.OracleSystem(*shape)
+    o.random_init(5, S0)
+    o.wall(2, 0, 1, 5.3, S0)
+    o.sphere((4.0, 4.0, 3.5), 2.0, 0, 3.1, S0)
+    o.set_params(A, B, C, 2.32, 1.0, 0.5, 0.2, 0.3, semantics=0)
+    o.set_efield((0.0, 0.2, 0.5), 1.0, 1.0, 1.0)
+    o.set_hfield((0.3, 0.0, -0.4), 1.0, 1.0, 0.5)
+    dens = o.energy_density()
+    total, _ = o.energy()
+    assert dens.shape == (o.n,)
+    np.testing.assert_allclose(dens.sum(), total, rtol=1e-12)
+    objects = o.types() > 0
+    assert objects.any() and np.all(dens[objects] == 0.0)
+    assert np.all(dens[~objects] != 0.0)
+
+    # locality: one site's Q changes its own density and its six neighbours'
+    q = o.q()
+    i = 7 + 9 * (1 + 8 * 5)       # (7, 1, 5): a bulk site away from the objects
+    assert o.types()[i] == 0
+    q2 = q.copy()
+    q2[i] += np.array([0.01, -0.02, 0.03, 0.005, -0.01])
+    changed = set(np.where(o.energy_density(q2) != dens)[0].tolist())
+    nbrs = set(o.neighbors()[i].tolist())
+    assert changed == {i} | nbrs
+
+    # uniform state, no fields: the phase density alone, from the 3x3 matrix
+    o2 = po.OracleSystem(*shape)
+    n_dir = np.array([0.3, -0.5, 0.8]) / np.linalg.norm([0.3, -0.5, 0.8])
+    Qm = S0 * (np.outer(n_dir, n_dir) - np.eye(3) / 3.0)
+    q5 = np.array([Qm[0, 0], Qm[0, 1], Qm[0, 2], Qm[1, 1], Qm[1, 2]])
+    o2.set_q(np.tile(q5, (o2.n, 1)))
+    o2.set_params(A, B, C, 2.32, 1.0, 0.5, 0.2, 0.3, semantics=0)
+    M = _q_matrix(q5)
+    t2, t3 = np.trace(M @ M), np.trace(M @ M @ M)
+    want = 0.5 * A * t2 + B / 3.0 * t3 + 0.25 * C * t2 * t2
+    np.testing.assert_allclose(o2.energy_density(), want, rtol=1e-13)
