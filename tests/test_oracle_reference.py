@@ -453,3 +453,62 @@ def test_energy_density_sums_to_total_and_is_local():
     t2, t3 = np.trace(M @ M), np.trace(M @ M @ M)
     want = 0.5 * A * t2 + B / 3.0 * t3 + 0.25 * C * t2 * t2
     np.testing.assert_allclose(o2.energy_density(), want, rtol=1e-13)
+
+
+def _rotated_state(q, L, axis):
+    """The periodic L^3 state q (site order, x fastest) rotated by 90 degrees about
+    `axis`: site r goes to R r (mod L) and Q to R Q R^T.  Returns (q', R, T), with
+    T the 5x5 map of the (Qxx, Qxy, Qxz, Qyy, Qyz) components."""
+    c = [(1, 2), (2, 0), (0, 1)][axis]          # the rotated coordinate pair (a, b) -> (-b, a)
+    R = np.eye(3)
+    R[c[0], c[0]] = R[c[1], c[1]] = 0.0
+    R[c[0], c[1]] = -1.0
+    R[c[1], c[0]] = 1.0
+    T = np.zeros((5, 5))
+    for k in range(5):
+        e = np.zeros(5)
+        e[k] = 1.0
+        M = R @ _q_matrix(e) @ R.T
+        T[:, k] = [M[0, 0], M[0, 1], M[0, 2], M[1, 1], M[1, 2]]
+    z, y, x = np.meshgrid(np.arange(L), np.arange(L), np.arange(L), indexing="ij")
+    r = np.stack([x.ravel(), y.ravel(), z.ravel()])        # coordinates of sites 0..n-1
+    r2 = (R.astype(int) @ r) % L
+    dest = r2[0] + L * (r2[1] + L * r2[2])
+    q2 = np.empty_like(q)
+    q2[dest] = q @ T.T
+    return q2, dest, T
+
+
+@pytest.mark.parametrize("L4", [0.0, 0.2])
+def test_energy_and_l1_l4_forces_are_covariant_under_lattice_rotations(L4):
+    """Every distortion term of the reference is a rotational invariant (L4, the
+    chiral one, under proper rotations), and its discrete energy uses central
+    differences, which 90-degree lattice rotations map onto each other: on a
+    periodic cube with a random Q field and no objects the energy with all of
+    L1..L6 is unchanged by a rotation about each axis.  The L1 and L4 forces
+    (with the phase force, metric corrected once as on the reference's GPU path)
+    rotate with the field, F'(R r) = T F(r).  A transposed operand, a wrong axis
+    or a sign in an energy polynomial or in those stencils breaks this.  (The
+    reference's L2, L3 and L6 force stencils, which the oracle reproduces
+    bit-exactly, are not covariant: they differ from a rotated field's by about
+    1e-2 of the force scale, so they are not tested this way.)"""
+    A, B, C = po.rescaled_phase()
+    S0 = po.uniaxial_s0(A, B, C)
+    L = 6
+    o = po.OracleSystem(L, L, L)
+    o.random_init(11, S0)
+    o.set_params(A, B, C, 2.32, 1.0, 0.5, L4, 0.3, multi=True, semantics=1)
+    q = o.q()
+    e0, c0 = o.energy(q)
+    for axis in range(3):
+        q2, dest, T = _rotated_state(q, L, axis)
+        e1, c1 = o.energy(q2)
+        np.testing.assert_allclose(e1, e0, rtol=1e-12)
+        np.testing.assert_allclose(c1, c0, rtol=1e-12, atol=1e-12 * abs(e0))
+    o.set_params(A, B, C, 2.32, 0.0, 0.0, L4, 0.0, multi=True, semantics=1)
+    f0 = o.forces(q)
+    scale = np.abs(f0).max()
+    for axis in range(3):
+        q2, dest, T = _rotated_state(q, L, axis)
+        f1 = o.forces(q2)
+        np.testing.assert_allclose(f1[dest], f0 @ T.T, rtol=0, atol=1e-12 * scale)
