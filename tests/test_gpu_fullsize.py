@@ -5,6 +5,8 @@ not depend on size (the oracle checks the same kernels at small sizes):
     shifted forces, bit for bit (the per-site arithmetic does not depend on where
     a site falls in a tile, a z chunk or the grid);
   * a uniform nematic at S0 is stationary (no force, FIRE stops at once);
+  * with the phase term off, an L1 plane wave along any axis has the closed-form
+    force -4 L1 (1 - cos k) Q (one-constant and composed multi-constant paths);
   * the energy is extensive: a 2x2x2 periodic copy of a 128^3 state has 8 times
     its energy;
   * FIRE keeps a 2x2x2 periodic copy exactly periodic, and follows the 128^3
@@ -49,6 +51,27 @@ def test_uniform_nematic_is_stationary_256():
     assert np.abs(F).max() < 1e-13
     st = qmin.FIRE(fc=1e-12, max_iter=100).minimize(f, lat)
     assert st.iterations == 1 and st.forceMax < 1e-12
+
+
+@pytest.mark.parametrize("multi", [False, True])
+def test_l1_plane_wave_closed_form_256(multi):
+    """Q(r) = eps sin(k r_axis) e, k = 2 pi 5 / 256, phase off: 6Q - (sum of
+    neighbours) = 2 (1 - cos k) Q, and the L1 shaping followed by the GPU path's
+    metric correction is twice the identity, so F = -4 L1 (1 - cos k) Q."""
+    L1 = 2.32
+    f = qmin.LandauDeGennesLC(0.0, 0.0, 0.0, L1, multi=multi, semantics=qmin.SEMANTICS_REF_GPU)
+    lat = qmin.Lattice(L, L, L)
+    e = np.array([0.3, -0.2, 0.5, 0.1, -0.4])
+    k = 2.0 * np.pi * 5 / L
+    wave = np.sin(k * np.arange(L))
+    for axis in range(3):
+        shape = [1, 1, 1]
+        shape[2 - axis] = L                     # (z, y, x) grid axes
+        q = np.broadcast_to((0.01 * wave).reshape(shape)[..., None] * e, (L, L, L, 5)).reshape(-1, 5)
+        lat.set_field(qmin.FIELD_Q, np.ascontiguousarray(q))
+        F = f.compute_forces(lat)
+        want = -4.0 * L1 * (1.0 - np.cos(k)) * q
+        assert np.abs(F - want).max() <= 1e-12 * np.abs(want).max(), axis
 
 
 def test_energy_is_extensive_256():

@@ -512,3 +512,28 @@ def test_energy_and_l1_l4_forces_are_covariant_under_lattice_rotations(L4):
         q2, dest, T = _rotated_state(q, L, axis)
         f1 = o.forces(q2)
         np.testing.assert_allclose(f1[dest], f0 @ T.T, rtol=0, atol=1e-12 * scale)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_l1_plane_wave_closed_form(axis):
+    """Closed form of the L1 force (lcForces.h:26-37 with the metric correction of
+    the GPU path, landauDeGennesLC.cpp:137-160): with the phase term off, a plane
+    wave Q(r) = eps sin(k r_axis) e on a periodic lattice (k = 2 pi m / L) has
+    6Q - (sum of neighbours) = 2 (1 - cos k) Q; the shaping (s0 += s0 + s3,
+    s3 += s0 + s3, the others doubled) followed by the metric (f0 <- 2/3 (2 f0 -
+    f3), f3 <- 2/3 (2 f3 - f0)) is twice the identity, so F = -4 L1 (1 - cos k) Q
+    exactly, along each axis."""
+    shape = (8, 10, 12)
+    L_axis = shape[axis]
+    o = po.OracleSystem(*shape)
+    L1 = 2.32
+    o.set_params(0.0, 0.0, 0.0, L1, semantics=1)
+    e = np.array([0.3, -0.2, 0.5, 0.1, -0.4])
+    z, y, x = np.meshgrid(np.arange(shape[2]), np.arange(shape[1]), np.arange(shape[0]), indexing="ij")
+    r = (x, y, z)[axis].ravel()
+    for m in (1, 3):
+        k = 2.0 * np.pi * m / L_axis
+        q = 0.01 * np.outer(np.sin(k * r), e)
+        f = o.forces(q)
+        want = -4.0 * L1 * (1.0 - np.cos(k)) * q
+        np.testing.assert_allclose(f, want, rtol=0, atol=1e-14 * np.abs(want).max())
