@@ -89,6 +89,7 @@ def oracle_lib():
         _sig(lib, "orc_first_derivatives", None, [_VP, _D, _D])
         _sig(lib, "orc_energy", _d, [_VP, _D, _D])
         _sig(lib, "orc_energy_density", None, [_VP, _D, _D])
+        _sig(lib, "orc_stress_tensor", None, [_D, _d, _d, _D])
         _sig(lib, "orc_fire", _i, [_VP, _d, _d, _d, _d, _d, _d, _i, _d, _d, _i, _D])
         _sig(lib, "orc_adam", _i, [_VP, _d, _d, _d, _d, _d, _i, _D])
         _sig(lib, "orc_nesterov", _i, [_VP, _d, _d, _d, _i, _i, _D])

@@ -892,6 +892,33 @@ void orc_energy_density(orc_system* s, const double* q, double* out) {
     }
 }
 
+// computeStressTensors, one-constant form (landauDeGennesLCOtherForces.cpp:214-251):
+// the stress of site derivatives fd = [dx q0..4, dy q0..4, dz q0..4] at elastic
+// constant L1, row-major m[9], with the site's energy density e added on the
+// diagonal.
+void orc_stress_tensor(const double* fd, double L1, double e, double* m) {
+    const double* X = fd;
+    const double* Y = fd + 5;
+    const double* Z = fd + 10;
+    auto diag = [&](const double* D) {
+        return -2 * L1 * (D[0] * D[0] + 2 * (D[1] * D[1]) + 2 * (D[2] * D[2]) + D[3] * D[3] + 2 * (D[4] * D[4]) +
+                          D[0] * D[3]);
+    };
+    auto off = [&](const double* P, const double* R) {
+        return L1 * (-(R[0] * (2 * P[0] + P[3])) - R[3] * (P[0] + 2 * P[3]) -
+                     4 * (R[1] * P[1] + R[2] * P[2] + R[4] * P[4]));
+    };
+    m[0] = diag(X) + e;
+    m[1] = off(X, Y);
+    m[2] = off(X, Z);
+    m[3] = off(X, Y);
+    m[4] = diag(Y) + e;
+    m[5] = off(Y, Z);
+    m[6] = off(X, Z);
+    m[7] = off(Y, Z);
+    m[8] = diag(Z) + e;
+}
+
 // landauDeGennesLC::computeObjectForces (landauDeGennesLCOtherForces.cpp:147-208)
 // with computeStressTensors (:214-251; one-constant only, as there): over the
 // object's surface sites, the stress tensor from the first derivatives
@@ -934,21 +961,12 @@ int orc_object_force(orc_system* s, int obj, double* out) {
         double et[5];
         distortionTerms(qc, d[0], d[1], d[2], et);
         e += L1 * et[0];
-        // stress (one-constant form), fd = [dx q0..4, dy q0..4, dz q0..4]
-        const double* X = d[0];
-        const double* Y = d[1];
-        const double* Z = d[2];
-        auto diag = [&](const double* D) {
-            return -2 * L1 * (D[0] * D[0] + 2 * (D[1] * D[1]) + 2 * (D[2] * D[2]) + D[3] * D[3] + 2 * (D[4] * D[4]) +
-                              D[0] * D[3]);
-        };
-        auto off = [&](const double* P, const double* R) {
-            return L1 * (-(R[0] * (2 * P[0] + P[3])) - R[3] * (P[0] + 2 * P[3]) -
-                         4 * (R[1] * P[1] + R[2] * P[2] + R[4] * P[4]));
-        };
-        const double m11 = diag(X) + e, m22 = diag(Y) + e, m33 = diag(Z) + e;
-        const double m12 = off(X, Y), m13 = off(X, Z), m21 = off(X, Y), m23 = off(Y, Z), m31 = off(X, Z),
-                     m32 = off(Y, Z);
+        double fd[15], m[9];
+        for (int ax = 0; ax < 3; ++ax)
+            for (int k = 0; k < 5; ++k) fd[5 * ax + k] = d[ax][k];
+        orc_stress_tensor(fd, L1, e, m);
+        const double m11 = m[0], m12 = m[1], m13 = m[2], m21 = m[3], m22 = m[4], m23 = m[5], m31 = m[6], m32 = m[7],
+                     m33 = m[8];
         double vx = 0, vy = 0, vz = 0;
         if (s->types[nb[0]] > 0) vx = -1.0;
         if (s->types[nb[1]] > 0) vx = 1.0;

@@ -64,6 +64,8 @@ void orc_first_derivatives(orc_system* s, const double* q, double* d15);
 double orc_energy(orc_system* s, const double* q, double* comps);
 // per-site energy density (landauDeGennesLC::energyDensity), n values; 0 on object sites
 void orc_energy_density(orc_system* s, const double* q, double* out);
+// one-constant stress tensor of derivatives fd[15] (computeStressTensors), row-major m[9], e on the diagonal
+void orc_stress_tensor(const double* fd, double L1, double e, double* m);
 // force callback usable by the reference harness (ctx = orc_system*)
 void orc_force_cb(void* ctx, int n, const double* q, double* f);
 double orc_energy_cb(void* ctx, int n, const double* q);

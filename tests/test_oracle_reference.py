@@ -7,6 +7,7 @@ reference (oracle/_ref) is present the same checks also run live.
 import ctypes
 import json
 import os
+import re
 
 import numpy as np
 import pytest
@@ -537,3 +538,46 @@ def test_l1_plane_wave_closed_form(axis):
         f = o.forces(q)
         want = -4.0 * L1 * (1.0 - np.cos(k)) * q
         np.testing.assert_allclose(f, want, rtol=0, atol=1e-14 * np.abs(want).max())
+
+
+REF_STRESS_SRC = "/root/reference/src/forces/landauDeGennesLCOtherForces.cpp"
+
+
+@pytest.mark.skipif(not os.path.exists(REF_STRESS_SRC), reason="reference checkout absent")
+def test_stress_tensor_matches_reference_expressions():
+    """The oracle's one-constant stress tensor (computeObjectForces' per-site
+    matrix) against the reference's own nine expressions
+    (landauDeGennesLCOtherForces.cpp:214-251), read from its source text and
+    evaluated on random derivative vectors, with the energy density added on the
+    diagonal as there."""
+    src = open(REF_STRESS_SRC).read()
+    body = src[src.index("void landauDeGennesLC::computeStressTensors"):]
+    call = body[body.index("stress.data[ii].set(") + len("stress.data[ii].set("):]
+    depth, parts, cur = 0, [], ""
+    for ch in call:
+        if ch == "(":
+            depth += 1
+        elif ch == ")":
+            if depth == 0:
+                parts.append(cur)
+                break
+            depth -= 1
+        if ch == "," and depth == 0:
+            parts.append(cur)
+            cur = ""
+            continue
+        cur += ch
+    assert len(parts) == 9
+    exprs = [compile(re.sub(r"firstDerivative\[(\d+)\]", r"d[\1]", p.strip()), "x%d" % k, "eval")
+             for k, p in enumerate(parts)]
+    lib = po.oracle_lib()
+    rng = np.random.default_rng(9)
+    for _ in range(50):
+        d = np.ascontiguousarray(rng.standard_normal(15))
+        L1 = float(rng.uniform(0.5, 3.0))
+        e = float(rng.standard_normal())
+        m = np.zeros(9)
+        lib.orc_stress_tensor(po.dptr(d), L1, e, po.dptr(m))
+        want = np.array([eval(x, {"L1": L1, "d": d}) for x in exprs])
+        want[[0, 4, 8]] += e
+        np.testing.assert_allclose(m, want, rtol=1e-13, atol=1e-13 * np.abs(want).max())
