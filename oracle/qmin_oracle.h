@@ -13,7 +13,12 @@
 // Parity pins (see tests/): per-site stencils and anchoring against the
 // compiled reference (oracle/_ref), the minimizer loops against the compiled
 // reference minimizers driven with this oracle's forces, lattice/types/init
-// against the compiled reference model, and closed forms for the rest.
+// against the compiled reference model, energy and stress polynomials against
+// the reference's expressions read from its source, and closed forms and
+// invariances (plane wave, lattice rotations, locality) for the rest.
+// Parity unpinned: orc_object_force's per-site area vector and sum (checked by
+// symmetry and translation properties only; the reference's force class that
+// computes them needs MPI and is not compiled here).
 #pragma once
 #include <stdint.h>
 
