@@ -319,7 +319,8 @@ class landauDeGennesLC {
     }
     // landauDeGennesLC::computeEnergy: the total (not per site); fills
     // energyComponents {phase, distortion, anchoring, E, H} and, as the
-    // reference does, the per-site energyDensity
+    // reference does, the per-site energyDensity (n doubles copied to the host
+    // on every call)
     scalar computeEnergy(bool verbose = false) {
         double total = 0;
         energyComponents.assign(5, 0.0);
