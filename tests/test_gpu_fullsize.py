@@ -6,7 +6,8 @@ not depend on size (the oracle checks the same kernels at small sizes):
     a site falls in a tile, a z chunk or the grid);
   * a uniform nematic at S0 is stationary (no force, FIRE stops at once);
   * with the phase term off, an L1 plane wave along any axis has the closed-form
-    force -4 L1 (1 - cos k) Q (one-constant and composed multi-constant paths);
+    force -4 L1 (1 - cos k) Q (one-constant and composed multi-constant paths),
+    also at 512^3, where field offsets pass 2^32 bytes;
   * the energy is extensive: a 2x2x2 periodic copy of a 128^3 state has 8 times
     its energy;
   * FIRE keeps a 2x2x2 periodic copy exactly periodic, and follows the 128^3
@@ -71,7 +72,30 @@ def test_l1_plane_wave_closed_form_256(multi):
         lat.set_field(qmin.FIELD_Q, np.ascontiguousarray(q))
         F = f.compute_forces(lat)
         want = -4.0 * L1 * (1.0 - np.cos(k)) * q
-        assert np.abs(F - want).max() <= 1e-12 * np.abs(want).max(), axis
+        assert np.abs(F - want).max() <= 1e-13 * L1 * np.abs(q).max(), axis
+
+
+def test_l1_plane_wave_closed_form_512():
+    """The same closed form on a 512^3 lattice (134M sites): each FP64 field then
+    spans 5.4 GB, so element offsets pass 2^31 and byte offsets 2^32, and the
+    pass's 64-bit addressing is exercised end to end."""
+    n = 512
+    L1 = 2.32
+    f = qmin.LandauDeGennesLC(0.0, 0.0, 0.0, L1, multi=True, semantics=qmin.SEMANTICS_REF_GPU)
+    lat = qmin.Lattice(n, n, n)
+    e = np.array([0.3, -0.2, 0.5, 0.1, -0.4])
+    k = 2.0 * np.pi * 7 / n
+    wave = 0.01 * np.sin(k * np.arange(n))
+    q = np.empty((n, n, n, 5))
+    q[...] = wave[:, None, None, None] * e          # varies along z, the slowest axis
+    lat.set_field(qmin.FIELD_Q, q.reshape(-1, 5))
+    F = f.compute_forces(lat).reshape(n, n, n, 5)
+    c = -4.0 * L1 * (1.0 - np.cos(k))
+    # rounding of 6Q - (sum of neighbours) scales with the input, not with the
+    # (1 - cos k)-small result; a wrong site would be off by ~|c q|, ~1e11 times this bound
+    scale = L1 * np.abs(q).max()
+    for z in range(0, n, 64):                      # plane by plane, to bound host temporaries
+        assert np.abs(F[z:z + 64] - c * q[z:z + 64]).max() <= 1e-13 * scale, z
 
 
 def test_energy_is_extensive_256():
