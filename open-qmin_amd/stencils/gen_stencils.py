@@ -184,7 +184,9 @@ def shaped_laplacian():
     return {k: v for k, v in out.items() if v != 0}
 
 
-FENCE_EVERY = int(os.environ.get("QMIN_FENCE_EVERY", "4"))
+# scheduling fence every 4 stencil points in the linear operators: bounds the
+# values in flight (registers) without serialising the loads
+FENCE_EVERY = 4
 
 
 def gen_fast(tables):
