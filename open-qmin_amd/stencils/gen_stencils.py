@@ -8,15 +8,11 @@ derivative order d/dx, d/dy, d/dz of q0..q4).
 
 Output: ../csrc/gen/ldg_stencils.inc with
 
-  ldg_bulk_lin_q(q, acc)                  centre-site part of the bulk L2/L3/L4 forms
-  ldg_bulk_lin_n<d>(nq, dA, dB, acc)      part of neighbour d (its Q and its two
-                                          transverse derivative vectors A < B)
   ldg_boundary_lin(bcase, x, acc)         the 27 boundary-case L2/L3/L4 forms
   ldg_boundary_l6(bcase, x, out)          the 27 boundary-case L6 forms
 
 acc[t][c] accumulates term t (0 = L2, 1 = L3, 2 = L4) of component c at unit
-constant; the caller scales by L.  The bulk forms use no derivative along a
-neighbour's own axis; the generator checks that.
+constant; the caller scales by L.
 
 It also composes, for a site whose neighbours all have central derivatives,
 the bulk L1 (shaped Laplacian), L2, L3 and L4 forms with those central
@@ -47,47 +43,6 @@ def coef_literal(c):
     if "e" not in r and "." not in r:
         r += ".0"
     return "R(%s)" % r
-
-
-def split_input(i):
-    """Classify input i: ('q', c) | ('n', d, c) | ('d', d, axis, c)."""
-    if i < 5:
-        return ("q", i)
-    if i < 35:
-        return ("n", (i - 5) // 5, (i - 5) % 5)
-    j = i - 35
-    return ("d", j // 15, (j % 15) // 5, j % 5)
-
-
-def gen_bulk_linear(tables):
-    out = []
-    # centre
-    lines = ["template <typename R>",
-             "__device__ __forceinline__ void ldg_bulk_lin_q(const R* q, R (&acc)[3][5]) {"]
-    for t, name in enumerate(LIN_TERMS):
-        for e in tables[name]["bulk"]:
-            kind = split_input(e[2])
-            if kind[0] == "q":
-                lines.append("    acc[%d][%d] += %s * q[%d];" % (t, e[0], coef_literal(e[1]), kind[1]))
-    lines.append("}")
-    out += lines
-    for d in range(6):
-        axis = d // 2
-        trans = [a for a in range(3) if a != axis]
-        lines = ["template <typename R>",
-                 "__device__ __forceinline__ void ldg_bulk_lin_n%d(const R* nq, const R* dA, const R* dB, R (&acc)[3][5]) {" % d]
-        for t, name in enumerate(LIN_TERMS):
-            for e in tables[name]["bulk"]:
-                kind = split_input(e[2])
-                if kind[0] == "n" and kind[1] == d:
-                    lines.append("    acc[%d][%d] += %s * nq[%d];" % (t, e[0], coef_literal(e[1]), kind[2]))
-                elif kind[0] == "d" and kind[1] == d:
-                    assert kind[2] != axis, "bulk form uses a derivative along the neighbour's own axis"
-                    arr = "dA" if kind[2] == trans[0] else "dB"
-                    lines.append("    acc[%d][%d] += %s * %s[%d];" % (t, e[0], coef_literal(e[1]), arr, kind[3]))
-        lines.append("}")
-        out += lines
-    return out
 
 
 def xname(i):
@@ -358,8 +313,6 @@ def main():
     tables, cases = doc["tables"], doc["cases"]
     body = ["// Generated by open-qmin_amd/stencils/gen_stencils.py from ldg_stencil_tables.json.",
             "// Do not edit by hand.", "#pragma once", ""]
-    body += gen_bulk_linear(tables)
-    body += [""]
     body += gen_boundary(tables, cases)
     fast_dev, fast_host, lin_keys = gen_fast(tables)
     body += [""] + fast_dev
