@@ -581,3 +581,39 @@ def test_stress_tensor_matches_reference_expressions():
         want = np.array([eval(x, {"L1": L1, "d": d}) for x in exprs])
         want[[0, 4, 8]] += e
         np.testing.assert_allclose(m, want, rtol=1e-13, atol=1e-13 * np.abs(want).max())
+
+
+def test_first_derivative_rules_on_a_quadratic_field():
+    """computeFirstDerivatives (landauDeGennesLCOtherForces.cpp:15-145) on a field
+    quadratic along z, Q(z) = z^2 e, with walls at z = 0, 4, 6 and 8: central
+    differences in the bulk (z = 2); next to an object below (z = 1), the forward
+    difference Q(z+1) - Q(z); next to an object above (z = 3), and with objects on
+    both sides (z = 5, the upper one taking precedence), the backward difference
+    Q(z) - Q(z-1); zero along x and y, in which the field is uniform.
+    The values come from the field, so a wrong axis, side or sign fails."""
+    A, B, C = po.rescaled_phase()
+    S0 = po.uniaxial_s0(A, B, C)
+    lx, ly, lz = 4, 3, 10
+    o = po.OracleSystem(lx, ly, lz)
+    for zw in (0, 4, 6, 8):
+        o.wall(2, zw, 1, 0.0, S0)
+    o.set_params(A, B, C, 2.32, semantics=1)
+    e = np.array([0.3, -0.2, 0.5, 0.1, -0.4])
+    z = np.repeat(np.arange(lz), lx * ly).astype(float)
+    q = np.outer(z * z, e)
+    types = o.types()
+    lc = types <= 0
+    q[~lc] = 0.0                                  # object sites keep their own Q; not read here
+    d = o.first_derivatives(q).reshape(-1, 3, 5)
+    Qz = lambda zz: zz * zz * e
+    expect = {1: Qz(2) - Qz(1), 2: (Qz(3) - Qz(1)) / 2, 3: Qz(3) - Qz(2)}
+    for zz in (1, 2, 3):
+        i = int(np.where(z == zz)[0][0])
+        assert types[i] <= 0
+        np.testing.assert_allclose(d[i, 2], expect[zz], rtol=1e-15, atol=1e-15)
+        np.testing.assert_array_equal(d[i, 0], 0.0)
+        np.testing.assert_array_equal(d[i, 1], 0.0)
+    # z = 5: objects at 4 and 6 -> the upper one wins: Q(5) - Q(4), with Q(4) the object's Q
+    i5 = int(np.where(z == 5)[0][0])
+    i4 = int(np.where(z == 4)[0][0])
+    np.testing.assert_allclose(d[i5, 2], q[i5] - q[i4], rtol=1e-15, atol=1e-15)
