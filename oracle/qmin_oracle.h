@@ -15,7 +15,8 @@
 // reference minimizers driven with this oracle's forces, lattice/types/init
 // against the compiled reference model, energy and stress polynomials against
 // the reference's expressions read from its source, and closed forms and
-// invariances (plane wave, lattice rotations, locality) for the rest.
+// invariances (plane wave, lattice rotations, locality, first-derivative rules
+// on a quadratic field) for the rest.
 // Parity unpinned: orc_object_force's per-site area vector and sum (checked by
 // symmetry and translation properties only; the reference's force class that
 // computes them needs MPI and is not compiled here).
