@@ -80,11 +80,16 @@ static int build_descs(qm_ensemble* E, const qm_ldg* f, const qm_fire* params) {
     std::vector<int2> mlog, blog;
     for (int k = 0; k < count; ++k) {
         qm_lattice* L = E->lats[k];
+        // the ensemble's tile and z chunk live in its descriptors only: the
+        // lattice's own settings are restored once its arguments are filled
+        const int ownTile = L->tileOverride, ownZ = L->zchunk;
         L->tileOverride = tile;
         // whole columns (or chunks of 32 planes): an ensemble has work-groups enough
         L->zchunk = E->zchunk > 0 ? E->zchunk : L->zchunkOverride > 0 ? L->zchunkOverride : std::min(L->lz, 32);
         FireConst fc = fire_const(params, L);
         fill_args<R>(L, f, md[k], bd[k], fc);
+        L->tileOverride = ownTile;
+        L->zchunk = ownZ;
         md[k].gate = 0;
         bd[k].gate = 0;
         const int bb = boundary_blocks(L->nBoundary);
@@ -181,6 +186,8 @@ static int ensemble_fire(qm_ensemble* E, const qm_fire* params, qm_ldg* f, int g
         if (fmOut) fmOut[k] = st[k].forceMax;
         if (int rc = sync_ctrl_to_host(E->lats[k])) return rc;
         nan |= E->lats[k]->hCtrl.nanFlag;
+        // the deferred FIRE velocity mix, made concrete as after qm_fire_minimize
+        if (int rc = materialize(E->lats[k])) return rc;
     }
     if (nan) return fail(QM_ENAN, "FIRE produced a NaN force norm on an ensemble lattice");
     return QM_OK;

@@ -553,7 +553,7 @@ static int drive(VecMinimizer& s, qm_lattice* L, Enqueue&& enqueue) {
     if (s.traceCap > 0 && h.traceLen > 0) {
         const size_t old = s.trace.size();
         s.trace.resize(old + (size_t)h.traceLen);
-        HIP_TRY(hipMemcpy(s.trace.data() + old, s.dTrace, (size_t)h.traceLen * sizeof(double),
+        HIP_TRY(copy_on(L, s.trace.data() + old, s.dTrace, (size_t)h.traceLen * sizeof(double),
                           hipMemcpyDeviceToHost));
     }
     if (h.nanFlag) return fail(QM_ENAN, "minimizer produced a NaN force norm");

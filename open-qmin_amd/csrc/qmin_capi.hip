@@ -95,6 +95,14 @@ int qmin::sync_ctrl_to_host(qm_lattice* L) {
     return QM_OK;
 }
 
+// Host <-> device copy ordered on the lattice's stream (which does not
+// synchronize with the null stream), completed before returning.
+hipError_t qmin::copy_on(qm_lattice* L, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, L->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(L->stream);
+    return e;
+}
+
 int qmin::push_ctrl(qm_lattice* L) {
     HIP_TRY(hipMemcpyAsync(L->dCtrl, &L->hCtrl, sizeof(Ctrl), hipMemcpyHostToDevice, L->stream));
     HIP_TRY(hipStreamSynchronize(L->stream));
@@ -139,17 +147,17 @@ static int refresh_types(qm_lattice* L) {
                 if (c < 0) sites.push_back((int32_t)i);
                 active += t <= 0;
             }
-    HIP_TRY(hipMemcpy(L->dTypes, t8.data(), n, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(L->dClass, cls.data(), n, hipMemcpyHostToDevice));
+    HIP_TRY(copy_on(L, L->dTypes, t8.data(), n, hipMemcpyHostToDevice));
+    HIP_TRY(copy_on(L, L->dClass, cls.data(), n, hipMemcpyHostToDevice));
     std::vector<DevObject> objs(kMaxObjects);
     for (size_t j = 0; j < L->objects.size(); ++j) objs[j] = {L->objects[j].type, 0, L->objects[j].W, L->objects[j].S0};
-    HIP_TRY(hipMemcpy(L->dObjects, objs.data(), sizeof(DevObject) * kMaxObjects, hipMemcpyHostToDevice));
+    HIP_TRY(copy_on(L, L->dObjects, objs.data(), sizeof(DevObject) * kMaxObjects, hipMemcpyHostToDevice));
     if (L->dSites) HIP_TRY(hipFree(L->dSites));
     L->dSites = nullptr;
     L->nBoundary = (long long)sites.size();
     if (!sites.empty()) {
         HIP_TRY(hipMalloc(&L->dSites, sites.size() * sizeof(int32_t)));
-        HIP_TRY(hipMemcpy(L->dSites, sites.data(), sites.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        HIP_TRY(copy_on(L, L->dSites, sites.data(), sites.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     }
     L->nActive = active;
     L->typesDirty = false;
@@ -172,13 +180,13 @@ static int upload_field(qm_lattice* L, int which, const double* aos) {
         for (long long i = 0; i < n; ++i)
             for (int c = 0; c < 5; ++c) soa[c * n + i] = aos[5 * i + c];
         double* base = static_cast<double*>(L->dSet[L->hCtrl.cur]) + which * 5 * n;
-        HIP_TRY(hipMemcpy(base, soa.data(), 5 * n * sizeof(double), hipMemcpyHostToDevice));
+        HIP_TRY(copy_on(L, base, soa.data(), 5 * n * sizeof(double), hipMemcpyHostToDevice));
     } else {
         std::vector<float> soa(5 * n);
         for (long long i = 0; i < n; ++i)
             for (int c = 0; c < 5; ++c) soa[c * n + i] = (float)aos[5 * i + c];
         float* base = static_cast<float*>(L->dSet[L->hCtrl.cur]) + which * 5 * n;
-        HIP_TRY(hipMemcpy(base, soa.data(), 5 * n * sizeof(float), hipMemcpyHostToDevice));
+        HIP_TRY(copy_on(L, base, soa.data(), 5 * n * sizeof(float), hipMemcpyHostToDevice));
     }
     return QM_OK;
 }
@@ -189,13 +197,13 @@ static int download_field(qm_lattice* L, int which, double* aos) {
     if (L->precision == QM_FP64) {
         std::vector<double> soa(5 * n);
         const double* base = static_cast<double*>(L->dSet[L->hCtrl.cur]) + which * 5 * n;
-        HIP_TRY(hipMemcpy(soa.data(), base, 5 * n * sizeof(double), hipMemcpyDeviceToHost));
+        HIP_TRY(copy_on(L, soa.data(), base, 5 * n * sizeof(double), hipMemcpyDeviceToHost));
         for (long long i = 0; i < n; ++i)
             for (int c = 0; c < 5; ++c) aos[5 * i + c] = soa[c * n + i];
     } else {
         std::vector<float> soa(5 * n);
         const float* base = static_cast<float*>(L->dSet[L->hCtrl.cur]) + which * 5 * n;
-        HIP_TRY(hipMemcpy(soa.data(), base, 5 * n * sizeof(float), hipMemcpyDeviceToHost));
+        HIP_TRY(copy_on(L, soa.data(), base, 5 * n * sizeof(float), hipMemcpyDeviceToHost));
         for (long long i = 0; i < n; ++i)
             for (int c = 0; c < 5; ++c) aos[5 * i + c] = soa[c * n + i];
     }
@@ -310,7 +318,7 @@ int qm_lattice_set_hfield(qm_lattice* L, const double* h3) {
     for (long long i = 0; i < n; ++i)
         for (int a = 0; a < 3; ++a) planes[(size_t)a * n + i] = h3[3 * i + a];
     if (!L->dHField) HIP_TRY(hipMalloc(&L->dHField, 3 * (size_t)n * sizeof(double)));
-    HIP_TRY(hipMemcpy(L->dHField, planes.data(), 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(copy_on(L, L->dHField, planes.data(), 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice));
     return QM_OK;
 }
 
@@ -835,11 +843,11 @@ static int upload_weights(qm_lattice* L, const qm_ldg* f) {
     if (L->precision == QM_FP64) {
         double w[kFastWeights];
         composed_weights<double>(f, w);
-        HIP_TRY(hipMemcpy(L->dW, w, sizeof(w), hipMemcpyHostToDevice));
+        HIP_TRY(copy_on(L, L->dW, w, sizeof(w), hipMemcpyHostToDevice));
     } else {
         float w[kFastWeights];
         composed_weights<float>(f, w);
-        HIP_TRY(hipMemcpy(L->dW, w, sizeof(w), hipMemcpyHostToDevice));
+        HIP_TRY(copy_on(L, L->dW, w, sizeof(w), hipMemcpyHostToDevice));
     }
     return QM_OK;
 }
@@ -1353,9 +1361,15 @@ static int fire_drive(qm_fire* m, qm_ldg* f, qm_lattice* L, int maxIter, bool to
     if (m->traceCap > 0 && h.traceLen > 0) {
         const size_t old = m->trace.size();
         m->trace.resize(old + 6 * (size_t)h.traceLen);
-        HIP_TRY(hipMemcpy(m->trace.data() + old, L->dTrace, 6 * (size_t)h.traceLen * sizeof(double),
+        HIP_TRY(copy_on(L, m->trace.data() + old, L->dTrace, 6 * (size_t)h.traceLen * sizeof(double),
                           hipMemcpyDeviceToHost));
     }
+    // energyMinimizerFIRE::fireStep mixes the velocity with the force of the
+    // step's end at once (energyMinimizerFIRE.cpp:113-116); the fused pass defers
+    // that mix to the next pass.  Make it concrete before returning, so that
+    // whatever changes Q or F between minimize calls (object moves, a new Q, a
+    // force computation) cannot change the mix.
+    if (int rc = materialize(L)) return rc;
     if (h.nanFlag) return fail(QM_ENAN, "FIRE produced a NaN force norm");
     return QM_OK;
 }

@@ -88,6 +88,8 @@ int sync_ctrl_to_host(qm_lattice* L);
 // makes a pending FIRE velocity mix concrete in U of the current state set
 int materialize(qm_lattice* L);
 int push_ctrl(qm_lattice* L);
+// synchronous copy ordered after the work queued on the lattice's stream
+hipError_t copy_on(qm_lattice* L, void* dst, const void* src, size_t bytes, hipMemcpyKind kind);
 int ensure_partials(qm_lattice* L, size_t count);
 int64_t n_active(const qm_lattice* L);
 // stencil variant of a force computer (kOneConstant .. kMultiL6)

@@ -596,7 +596,7 @@ int qm_ws_features(qm_lattice* L, double dx, double dy, double dz, void* out) {
 // the device site types (int8, reference encoding clamped to the object index)
 int qm_ws_types(qm_lattice* L, void* out) {
     if (!L || !out) return fail(QM_EINVAL, "bad argument");
-    HIP_TRY(hipMemcpy(out, L->dTypes, L->n, hipMemcpyDeviceToDevice));
+    HIP_TRY(copy_on(L, out, L->dTypes, L->n, hipMemcpyDeviceToDevice));
     return QM_OK;
 }
 

@@ -256,6 +256,30 @@ def test_object_sites_displacement_and_force_match_oracle():
         multi.object_force(g, 2)
 
 
+@pytest.mark.parametrize("semantics,params", [(1, ONE), (0, MULTI)])
+def test_fire_displace_fire_matches_oracle(semantics, params):
+    """The moving-colloid workflow: minimize, move an object, minimize again.
+    energyMinimizerFIRE mixes the velocity with the force at the end of each step
+    (energyMinimizerFIRE.cpp:113-116), so the velocity that the second run starts
+    from holds the force from before the move, not after it."""
+    o, g, f = build_pair((18, 16, 14), params, semantics, True)
+    qmin.FIRE(max_iter=20).minimize(f, g)
+    o.fire(max_iter=20)
+    # (no read-back of v here: reading it would make the mix concrete itself)
+    g.displace_object(2, 1, 1)
+    o.displace_object(2, 1, 1)
+    assert np.array_equal(g.types(), o.types())
+    # a force evaluation in between changes nothing either
+    f.compute_forces(g)
+    m = qmin.FIRE(max_iter=20)
+    m.set_trace(32)
+    m.minimize(f, g)
+    it_ref, tr_ref = o.fire(max_iter=20)
+    np.testing.assert_allclose(m.trace()[:, 0], tr_ref[:, 0], rtol=1e-10)
+    np.testing.assert_allclose(g.q(), o.q(), rtol=0, atol=1e-11)
+    np.testing.assert_allclose(g.v(), o.v(), rtol=0, atol=1e-11)
+
+
 def test_fire_continues_across_calls():
     """Two minimize calls of 20 iterations equal one of 40 (state persists on the device)."""
     o, g, f = build_pair((18, 14, 12), MULTI, 0, True)
