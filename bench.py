@@ -281,27 +281,56 @@ def measured_traffic(size, precision):
     return d["read_bytes"] + d["write_bytes"]
 
 
-def cpu_baseline(L=128, iters=36):
-    """The CPU oracle on one host core: same physics and geometry scaled to L^3."""
+def host_cores():
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(L=256, threads_all=None, reps=3):
+    """The CPU oracle (a restatement of the reference CPU path, which is
+    single-threaded) on this workload at full size: one FIRE iteration is the
+    time of fire(max_iter=1) (one force evaluation and one iteration) minus one
+    force evaluation, each the median of `reps` runs (about 45 s on one
+    core at 256^3); on one core, and with
+    OpenMP on every host core this process may use (the reference has no
+    threading; that row is for comparison)."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import pyoracle as po
     A, B, C, S0 = phase_constants()
     o = po.OracleSystem(L, L, L)
-    o.threads(1)
     o.random_init(1234, S0)
     o.wall(2, 0, 1, 5.3, S0)
     o.wall(2, L - 1, 1, 5.3, S0)
     o.sphere((L / 2.0, L / 2.0, L / 2.0), 20.0 * L / 256.0, 0, 5.3, S0)
     o.set_params(A, B, C, 2.32, 0.5, 0.3, semantics=1)
-    o.fire(max_iter=1)  # warm caches
-    t0 = time.perf_counter()
-    o.fire(max_iter=iters)
-    t1 = time.perf_counter()
-    rate = L ** 3 * iters / (t1 - t0)
-    return dict(value=rate, unit="site-updates/s", cores=1, kind="port",
-                sample="%d FIRE iterations (+1 force evaluation) on a %d^3 lattice with the same walls, scaled "
-                       "sphere and L1-L3 constants, FP64, oracle/qmin_oracle.cpp single-threaded; %.1f s"
-                       % (iters, L, t1 - t0))
+    q0 = o.q()
+    all_cores = threads_all or host_cores()
+    rows = {}
+    t_start = time.perf_counter()
+    for threads in (1, all_cores):
+        o.threads(threads)
+        t_force, t_fire = [], []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            o.forces()
+            t_force.append(time.perf_counter() - t0)
+            o.set_q(q0)
+            t0 = time.perf_counter()
+            o.fire(max_iter=1)
+            t_fire.append(time.perf_counter() - t0)
+        per_it = sorted(t_fire)[reps // 2] - sorted(t_force)[reps // 2]
+        rows[threads] = dict(value=L ** 3 / per_it, s_per_iteration=per_it)
+    seconds = time.perf_counter() - t_start
+    one, many = rows[1], rows[all_cores]
+    return dict(value=one["value"], unit="site-updates/s", cores=1, kind="port",
+                s_per_iteration=one["s_per_iteration"], host_cores=host_cores(),
+                all_cores={"value": many["value"], "cores": all_cores, "s_per_iteration": many["s_per_iteration"]},
+                sample="the bench workload at %d^3 (walls, sphere, L1-L3, FP64): oracle/qmin_oracle.cpp, one FIRE "
+                       "iteration = fire(1 iteration) - one force evaluation, median of %d runs each, on 1 core "
+                       "(the reference CPU path is single-threaded) and with OpenMP on %d cores; %.1f s in all"
+                       % (L, reps, all_cores, seconds))
 
 
 def main():

@@ -310,6 +310,12 @@ int qm_ws_weights_to_bf16(const float* w, long long count, void* wbf16, void* wb
  * lossPart[blocks] gets per-block sums of squared differences */
 int qm_ws_mse_grad(const float* y, int cy, const float* target, const void* types, long long n, float scale,
                    const double* offset5, void* g, double* lossPart, int blocks, void* stream);
+/* energy term of the training loss: with the prediction in the lattice
+ * (qm_ws_apply), computes the forces F of force computer f there and adds
+ * -scale F to the first five of cy channels of g (BF16, device) at sites with
+ * type <= 0; F is minus the energy gradient shaped by the metric.  The
+ * caller's stream is drained first; the call returns when g is updated. */
+int qm_ws_energy_grad(qm_ldg* f, qm_lattice* lat, float scale, void* g, int cy, void* stream);
 /* [n][16] BF16 input features of the lattice's geometry and far-field director d */
 int qm_ws_features(qm_lattice* lat, double dx, double dy, double dz, void* out);
 int qm_ws_types(qm_lattice* lat, void* out);

@@ -11,7 +11,7 @@ OBJ = os.path.join(HERE, "build", "obj")
 LIB = os.path.join(HERE, "lib", "libqmin.so")
 DRIVER = os.path.join(HERE, "lib", "openQmin-hip")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
-SOURCES = ["ldg_kernels.hip", "qmin_capi.hip", "minimizers.hip", "ensemble.hip", "warmstart.hip"]
+SOURCES = ["ldg_kernels.hip", "fire_stream.hip", "qmin_capi.hip", "minimizers.hip", "ensemble.hip", "warmstart.hip"]
 # host code that must match the reference's g++ build bit for bit
 HOST_SOURCES = ["host_init.cpp"]
 GXX = os.environ.get("CXX", "g++")
@@ -57,6 +57,12 @@ def build(verbose=False):
          os.path.join(HERE, "host", "openqmin_hip.cpp"), "-o", DRIVER + ".tmp", "-L" + os.path.dirname(LIB),
          "-lqmin", "-Wl,-rpath,$ORIGIN"])
     os.replace(DRIVER + ".tmp", DRIVER)
+    # reference-style GPUArray / ArrayHandle use of the drop-in classes
+    example = os.path.join(os.path.dirname(LIB), "array-views-example")
+    run([GXX, "-std=c++17", "-O2", "-Wall", "-I" + INCLUDE, "-I" + os.path.join(HERE, "host"),
+         os.path.join(HERE, "host", "array_views_example.cpp"), "-o", example + ".tmp", "-L" + os.path.dirname(LIB),
+         "-lqmin", "-Wl,-rpath,$ORIGIN"])
+    os.replace(example + ".tmp", example)
     # streaming-ceiling measurement for the fused pass's access pattern
     ceiling = os.path.join(os.path.dirname(LIB), "qmin-stream-ceiling")
     run([HIPCC] + FLAGS + [os.path.join(CSRC, "stream_ceiling.hip"), "-o", ceiling + ".tmp"])

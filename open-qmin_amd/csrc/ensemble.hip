@@ -72,7 +72,8 @@ static std::vector<int2> xcd_order(const std::vector<int2>& logical) {
 template <typename R>
 static int build_descs(qm_ensemble* E, const qm_ldg* f, const qm_fire* params) {
     const int count = (int)E->lats.size();
-    const int tile = E->tile >= 0 ? E->tile : active_tile(E->lats[0]);
+    // batched passes have the tiles of the fused pass only
+    const int tile = E->tile >= 0 ? E->tile : active_tile(E->lats[0], kForce);
     E->usedTile = tile;
     std::vector<MarchArgs<R>> md(count);
     std::vector<BoundaryArgs<R>> bd(count);
@@ -87,7 +88,7 @@ static int build_descs(qm_ensemble* E, const qm_ldg* f, const qm_fire* params) {
         // whole columns (or chunks of 32 planes): an ensemble has work-groups enough
         L->zchunk = E->zchunk > 0 ? E->zchunk : L->zchunkOverride > 0 ? L->zchunkOverride : std::min(L->lz, 32);
         FireConst fc = fire_const(params, L);
-        fill_args<R>(L, f, md[k], bd[k], fc);
+        fill_args<R>(L, f, md[k], bd[k], fc, tile);
         L->tileOverride = ownTile;
         L->zchunk = ownZ;
         md[k].gate = 0;
@@ -221,7 +222,7 @@ static int ensemble_autotune(qm_ensemble* E, qm_ldg* f, int* tileOut, int* zchun
     double best = 1e300;
     int bestTile = 0, bestZ = 0, rc = QM_OK;
     const int reps = 3;
-    for (int tile = 0; tile < kNumTiles && rc == QM_OK; ++tile) {
+    for (int tile = 0; tile < kNumMarchTiles && rc == QM_OK; ++tile) {
         for (int zc : {8, 16, 32, 64}) {
             E->tile = tile;
             E->zchunk = zc;

@@ -29,6 +29,10 @@ void launch_fire_finalize(Ctrl* ctrl, const FireConst& fp, const double* partial
 template <typename R>
 int march_blocks_per_cu(int mode, int terms, int tile);
 template <typename R>
+hipError_t launch_fire_stream(const MarchArgs<R>& a, int terms, int tile, hipStream_t s);
+template <typename R>
+int stream_blocks_per_cu(int terms, int tile);
+template <typename R>
 void launch_materialize(StateSet<R> s0, StateSet<R> s1, long long n, Ctrl* ctrl, hipStream_t s);
 template <typename R>
 int launch_energy(const R* q, const int8_t* types, const DevObject* objects, int lx, int ly, int lz,
@@ -230,7 +234,7 @@ int qm_lattice_create(int lx, int ly, int lz, int precision, qm_lattice** out) {
     HIP_TRY(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking));
     HIP_TRY(hipMalloc(&L->dTypes, n));
     HIP_TRY(hipMalloc(&L->dClass, n));
-    HIP_TRY(hipMalloc(&L->dW, sizeof(double) * kFastWeights));
+    HIP_TRY(hipMalloc(&L->dW, sizeof(double) * 2 * kFastWeights));
     HIP_TRY(hipMalloc(&L->dObjects, sizeof(DevObject) * kMaxObjects));
     for (int s = 0; s < 2; ++s) {
         HIP_TRY(hipMalloc(&L->dSet[s], 15 * n * L->realSize));
@@ -342,7 +346,9 @@ int qm_lattice_load_hfield(qm_lattice* L, const char* path) {
 }
 
 int qm_lattice_set_tile(qm_lattice* L, int tile) {
-    if (!L || tile < -1 || tile >= kNumTiles) return fail(QM_EINVAL, "tile is -1 (automatic), 0 (32x8), 1 (64x4), 2 (32x16) or 3 (64x8)");
+    if (!L || tile < -1 || tile >= kNumTiles)
+        return fail(QM_EINVAL, "tile is -1 (automatic), 0 (32x8), 1 (64x4), 2 (32x16), 3 (64x8), or for FIRE "
+                               "passes that keep only Q and v 4 (32x16), 5 (32x8), 6 (64x8)");
     L->tileOverride = tile;
     return QM_OK;
 }
@@ -839,14 +845,35 @@ static void composed_weights(const qm_ldg* f, R* w) {
 }
 static_assert(kFastLinBCount <= kFastWeights, "composed operator larger than the weight buffer");
 
+// The device buffer holds the weights of the 19-point operator, then the few
+// distinct weights of its plane-by-plane form (each computed as the entries'
+// own weights are, so that both forms multiply by the same values).
+template <typename R>
+static void weight_buffer(const qm_ldg* f, R* w) {
+    composed_weights<R>(f, w);
+    R* plane = w + kFastWeights;
+    for (int k = 0; k < kFastWeights; ++k) plane[k] = R(0);
+    const int terms = terms_of(f);
+    if (terms == kOneConstant) return;
+    const double Ls[4] = {f->L1, f->L2, f->L3, f->L4};
+    const bool b = terms != kMultiLinear;
+    const int cnt = b ? kPlaneLinBCount : kPlaneLinACount;
+    for (int j = 0; j < cnt; ++j) {
+        const double* c = b ? kPlaneLinB[j] : kPlaneLinA[j];
+        double acc = 0;
+        for (int t = 0; t < 4; ++t) acc += Ls[t] * c[t];
+        plane[j] = R(acc);
+    }
+}
+
 static int upload_weights(qm_lattice* L, const qm_ldg* f) {
     if (L->precision == QM_FP64) {
-        double w[kFastWeights];
-        composed_weights<double>(f, w);
+        double w[2 * kFastWeights];
+        weight_buffer<double>(f, w);
         HIP_TRY(copy_on(L, L->dW, w, sizeof(w), hipMemcpyHostToDevice));
     } else {
-        float w[kFastWeights];
-        composed_weights<float>(f, w);
+        float w[2 * kFastWeights];
+        weight_buffer<float>(f, w);
         HIP_TRY(copy_on(L, L->dW, w, sizeof(w), hipMemcpyHostToDevice));
     }
     return QM_OK;
@@ -858,8 +885,13 @@ static int upload_weights(qm_lattice* L, const qm_ldg* f) {
 // (6% faster than 32 x 16, 11% faster than 64 x 4).  On smaller ones, and with
 // the L6 term (whose pass needs more than the 128 registers the 512-thread tiles
 // allow), 32 x 8 for FP64 and 64 x 4 for FP32.
-int qmin::active_tile(const qm_lattice* L) {
-    if (L->tileOverride >= 0) return L->tileOverride;
+// The tiles of the FIRE pass that keeps only Q and v (4..6) serve FIRE passes
+// without the L6 term; a force-only pass, or one with L6, then takes the
+// default of its kind.
+int qmin::active_tile(const qm_lattice* L, int mode) {
+    // the 64 x 8 FIRE tile that keeps only Q and v needs more LDS than a CU has in FP64
+    const int t = (L->tileOverride == kStream64x8 && L->precision == QM_FP64) ? kStream32x16 : L->tileOverride;
+    if (t >= 0 && !(is_stream_tile(t) && (mode != kFire || L->terms == kMultiL6))) return t;
     if (L->terms != kMultiL6 && L->lx >= 128 && L->ly >= 128)
         return L->precision == QM_FP64 ? kTile32x16 : kTile64x8;
     return L->precision == QM_FP64 ? kTile32x8 : kTile64x4;
@@ -870,7 +902,8 @@ long long qmin::tile_columns(const qm_lattice* L, int tile) {
 }
 
 template <typename R>
-void qmin::fill_args(qm_lattice* L, const qm_ldg* f, MarchArgs<R>& ma, BoundaryArgs<R>& ba, const FireConst& fire) {
+void qmin::fill_args(qm_lattice* L, const qm_ldg* f, MarchArgs<R>& ma, BoundaryArgs<R>& ba, const FireConst& fire,
+                     int tile) {
     ma.set[0] = state_set<R>(L, 0);
     ma.set[1] = state_set<R>(L, 1);
     ma.cls = L->dClass;
@@ -880,7 +913,6 @@ void qmin::fill_args(qm_lattice* L, const qm_ldg* f, MarchArgs<R>& ma, BoundaryA
     ma.lz = L->lz;
     ma.zchunk = std::min(L->zchunk, L->lz);
     ma.n = L->n;
-    const int tile = active_tile(L);
     ma.nbx = (L->lx + kTileTX[tile] - 1) / kTileTX[tile];
     ma.nby = (L->ly + kTileTY[tile] - 1) / kTileTY[tile];
     const int nbz = (L->lz + ma.zchunk - 1) / ma.zchunk;
@@ -911,9 +943,9 @@ void qmin::fill_args(qm_lattice* L, const qm_ldg* f, MarchArgs<R>& ma, BoundaryA
 }
 
 template void qmin::fill_args<double>(qm_lattice*, const qm_ldg*, MarchArgs<double>&, BoundaryArgs<double>&,
-                                      const FireConst&);
+                                      const FireConst&, int);
 template void qmin::fill_args<float>(qm_lattice*, const qm_ldg*, MarchArgs<float>&, BoundaryArgs<float>&,
-                                     const FireConst&);
+                                     const FireConst&, int);
 
 // z chunk of the fused pass: as long as possible while the grid still fills
 // every CU to its occupancy, so that the whole grid is resident at once, all
@@ -931,9 +963,14 @@ static void choose_zchunk(qm_lattice* L, int mode, int terms) {
             L->numCUs = prop.multiProcessorCount;
         if (L->numCUs <= 0) L->numCUs = 256;
     }
-    const int tile = active_tile(L);
-    const int perCU = L->precision == QM_FP64 ? march_blocks_per_cu<double>(mode, terms, tile)
-                                              : march_blocks_per_cu<float>(mode, terms, tile);
+    const int tile = active_tile(L, mode);
+    int perCU;
+    if (is_stream_tile(tile))
+        perCU = L->precision == QM_FP64 ? stream_blocks_per_cu<double>(terms, tile)
+                                        : stream_blocks_per_cu<float>(terms, tile);
+    else
+        perCU = L->precision == QM_FP64 ? march_blocks_per_cu<double>(mode, terms, tile)
+                                        : march_blocks_per_cu<float>(mode, terms, tile);
     const long long columns = tile_columns(L, tile);
     long long chunks = (long long)perCU * L->numCUs / columns;
     if (chunks < 1) chunks = 1;
@@ -941,9 +978,9 @@ static void choose_zchunk(qm_lattice* L, int mode, int terms) {
     L->zchunk = std::max(4, std::min(zc, L->lz));
 }
 
-static int march_blocks(const qm_lattice* L) {
+static int march_blocks(const qm_lattice* L, int mode) {
     const int zc = std::min(L->zchunk, L->lz);
-    return int(tile_columns(L, active_tile(L)) * ((L->lz + zc - 1) / zc));
+    return int(tile_columns(L, active_tile(L, mode)) * ((L->lz + zc - 1) / zc));
 }
 
 // One pass (force only or one FIRE iteration); kernels only, no host sync.
@@ -952,7 +989,8 @@ static int run_pass(qm_lattice* L, const qm_ldg* f, int mode, const FireConst& f
     choose_zchunk(L, mode, terms_of(f));
     MarchArgs<R> ma{};
     BoundaryArgs<R> ba{};
-    fill_args<R>(L, f, ma, ba, fire);
+    const int tile = active_tile(L, mode);
+    fill_args<R>(L, f, ma, ba, fire, tile);
     ma.gate = gated ? 1 : 0;
     ba.gate = ma.gate;
     const int terms = terms_of(f);
@@ -961,12 +999,15 @@ static int run_pass(qm_lattice* L, const qm_ldg* f, int mode, const FireConst& f
         for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
         HIP_TRY(hipEventRecord(ev[0], L->stream));
     }
-    HIP_TRY(launch_march<R>(ma, mode, terms, active_tile(L), L->stream));
+    if (is_stream_tile(tile))
+        HIP_TRY(launch_fire_stream<R>(ma, terms, tile, L->stream));
+    else
+        HIP_TRY(launch_march<R>(ma, mode, terms, tile, L->stream));
     if (g_timing.on && mode == kFire) HIP_TRY(hipEventRecord(ev[1], L->stream));
     launch_boundary<R>(ba, mode, terms, L->stream);
     HIP_TRY(hipGetLastError());
     if (mode == kFire) {
-        launch_fire_finalize(L->dCtrl, fire, L->dPartials, (long long)ma.nblocks + boundary_blocks(L->nBoundary),
+        launch_fire_finalize(L->dCtrl, fire, ma.partials, (long long)ma.nblocks + boundary_blocks(L->nBoundary),
                              L->dTrace, L->stream);
         HIP_TRY(hipGetLastError());
     }
@@ -994,14 +1035,18 @@ static int time_march(qm_lattice* L, const qm_ldg* f, int reps, double* ms) {
     MarchArgs<R> ma{};
     BoundaryArgs<R> ba{};
     FireConst fire{};
-    fill_args<R>(L, f, ma, ba, fire);
-    const int terms = terms_of(f), tile = active_tile(L);
-    for (int w = 0; w < 2; ++w) HIP_TRY(launch_march<R>(ma, kFire, terms, tile, L->stream));
+    const int terms = terms_of(f), tile = active_tile(L, kFire);
+    fill_args<R>(L, f, ma, ba, fire, tile);
+    auto launch = [&]() {
+        return is_stream_tile(tile) ? launch_fire_stream<R>(ma, terms, tile, L->stream)
+                                    : launch_march<R>(ma, kFire, terms, tile, L->stream);
+    };
+    for (int w = 0; w < 2; ++w) HIP_TRY(launch());
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     HIP_TRY(hipEventRecord(e0, L->stream));
-    for (int r = 0; r < reps; ++r) HIP_TRY(launch_march<R>(ma, kFire, terms, tile, L->stream));
+    for (int r = 0; r < reps; ++r) HIP_TRY(launch());
     HIP_TRY(hipEventRecord(e1, L->stream));
     HIP_TRY(hipEventSynchronize(e1));
     float t = 0;
@@ -1318,6 +1363,7 @@ static int fire_drive(qm_fire* m, qm_ldg* f, qm_lattice* L, int maxIter, bool to
         HIP_TRY(hipMalloc(&L->dTrace, 6 * (size_t)m->traceCap * sizeof(double)));
         L->traceCap = m->traceCap;
     }
+    const int traceCap = m->traceCap;
     if (int rc = sync_ctrl_to_host(L)) return rc;
     Ctrl& c = L->hCtrl;
     c.done = 0;
@@ -1329,20 +1375,22 @@ static int fire_drive(qm_fire* m, qm_ldg* f, qm_lattice* L, int maxIter, bool to
     c.arrive = 0;
     c.nanFlag = 0;
     c.traceLen = 0;
-    c.traceCap = m->traceCap;
+    c.traceCap = traceCap;
     c.forceMax = m->forceMax;
     if (int rc = push_ctrl(L)) return rc;
     FireConst fp = fire_const(m, L);
     if (!tolerance) fp.fc = -1.0;  // never converged: exactly maxIter - iterations steps
     // sim->computeForces() at the start of minimize (energyMinimizerFIRE.cpp:157)
-    int rc = L->precision == QM_FP64 ? run_pass<double>(L, f, kForce, fp) : run_pass<float>(L, f, kForce, fp);
+    int rc = L->precision == QM_FP64 ? run_pass<double>(L, f, kForce, fp)
+                                     : run_pass<float>(L, f, kForce, fp);
     if (rc) return rc;
     const int batch = L->n >= (1 << 22) ? 16 : 64;
     while (true) {
         int todo = std::min(batch, maxIter - L->hCtrl.iterations);
         if (todo <= 0) todo = 1;  // at least one iteration per minimize call
         for (int k = 0; k < todo; ++k) {
-            rc = L->precision == QM_FP64 ? run_pass<double>(L, f, kFire, fp) : run_pass<float>(L, f, kFire, fp);
+            rc = L->precision == QM_FP64 ? run_pass<double>(L, f, kFire, fp)
+                                         : run_pass<float>(L, f, kFire, fp);
             if (rc) return rc;
         }
         if ((rc = sync_ctrl_to_host(L))) return rc;
@@ -1358,18 +1406,25 @@ static int fire_drive(qm_fire* m, qm_ldg* f, qm_lattice* L, int maxIter, bool to
     m->scaling = h.scaling;
     m->nan = h.nanFlag;
     m->converged = !h.nanFlag && h.forceMax <= m->fc;
-    if (m->traceCap > 0 && h.traceLen > 0) {
+    if (traceCap > 0 && h.traceLen > 0) {
         const size_t old = m->trace.size();
         m->trace.resize(old + 6 * (size_t)h.traceLen);
         HIP_TRY(copy_on(L, m->trace.data() + old, L->dTrace, 6 * (size_t)h.traceLen * sizeof(double),
                           hipMemcpyDeviceToHost));
+    }
+    // The pass that keeps only Q and v leaves F stale on sites it finishes
+    // itself: recompute F of the final state, which the caller may read.
+    if (is_stream_tile(active_tile(L, kFire))) {
+        rc = L->precision == QM_FP64 ? run_pass<double>(L, f, kForce, fp)
+                                     : run_pass<float>(L, f, kForce, fp);
+        if (rc) return rc;
     }
     // energyMinimizerFIRE::fireStep mixes the velocity with the force of the
     // step's end at once (energyMinimizerFIRE.cpp:113-116); the fused pass defers
     // that mix to the next pass.  Make it concrete before returning, so that
     // whatever changes Q or F between minimize calls (object moves, a new Q, a
     // force computation) cannot change the mix.
-    if (int rc = materialize(L)) return rc;
+    if (int rc2 = materialize(L)) return rc2;
     if (h.nanFlag) return fail(QM_ENAN, "FIRE produced a NaN force norm");
     return QM_OK;
 }

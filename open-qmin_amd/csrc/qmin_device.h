@@ -11,11 +11,18 @@ namespace qmin {
 // whole cache lines for the side halo but have more halo sites per tile), and
 // 32 x 16 and 64 x 8 (512 threads, 73-79 KB of LDS in FP64: fewer halo sites per
 // site; with 64 x 8 each wave covers one contiguous 64-site row).
+// Tiles 4..6 are those of the FIRE pass that keeps only Q and v between
+// iterations (fire_stream.hip): TX x TY owned sites, one thread per site of the
+// tile plus a two-site margin.
 constexpr int kThreads = 256;   // work-group size of the streaming kernels
-enum TileId { kTile32x8 = 0, kTile64x4 = 1, kTile32x16 = 2, kTile64x8 = 3, kNumTiles = 4 };
-constexpr int kTileTX[kNumTiles] = {32, 64, 32, 64};
-constexpr int kTileTY[kNumTiles] = {8, 4, 16, 8};
-constexpr int kTileThreads[kNumTiles] = {32 * 8, 64 * 4, 32 * 16, 64 * 8};
+enum TileId {
+    kTile32x8 = 0, kTile64x4 = 1, kTile32x16 = 2, kTile64x8 = 3, kNumMarchTiles = 4,
+    kStream32x16 = 4, kStream32x8 = 5, kStream64x8 = 6, kNumTiles = 7
+};
+constexpr int kTileTX[kNumTiles] = {32, 64, 32, 64, 32, 32, 64};
+constexpr int kTileTY[kNumTiles] = {8, 4, 16, 8, 16, 8, 8};
+constexpr int kTileThreads[kNumMarchTiles] = {32 * 8, 64 * 4, 32 * 16, 64 * 8};
+__host__ __device__ constexpr bool is_stream_tile(int tile) { return tile >= kNumMarchTiles; }
 
 template <int TX_, int TY_>
 struct Tile {

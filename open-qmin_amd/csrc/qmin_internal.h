@@ -95,11 +95,12 @@ int64_t n_active(const qm_lattice* L);
 // stencil variant of a force computer (kOneConstant .. kMultiL6)
 int terms_of(const qm_ldg* f);
 // tile shape of the fused pass for this lattice, and its tile columns
-int active_tile(const qm_lattice* L);
+int active_tile(const qm_lattice* L, int mode);
 long long tile_columns(const qm_lattice* L, int tile);
 // kernel arguments of one lattice's passes (z chunk taken from L->zchunk)
 template <typename R>
-void fill_args(qm_lattice* L, const qm_ldg* f, MarchArgs<R>& ma, BoundaryArgs<R>& ba, const FireConst& fire);
+void fill_args(qm_lattice* L, const qm_ldg* f, MarchArgs<R>& ma, BoundaryArgs<R>& ba, const FireConst& fire,
+               int tile);
 FireConst fire_const(const qm_fire* m, const qm_lattice* L);
 int boundary_blocks(long long nsites);
 template <typename R>

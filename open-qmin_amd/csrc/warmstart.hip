@@ -362,6 +362,19 @@ __global__ void __launch_bounds__(kWsThreads) mse_grad_kernel(const float* y, in
     if (threadIdx.x == 0) lossPart[blockIdx.x] = red[0];
 }
 
+// Energy term of the training loss: g[s][c] -= scale F_c(s) for the first five
+// channels of liquid-crystal sites, F being the force computer's force on the
+// predicted state (minus the energy gradient, shaped by the metric).
+template <typename R>
+__global__ void __launch_bounds__(kWsThreads) energy_grad_kernel(const R* f, const int8_t* types, long long n,
+                                                               int cy, float scale, uint16_t* g) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        if (types[i] > 0) continue;
+#pragma unroll
+        for (int c = 0; c < 5; ++c) g[i * cy + c] = f2bf(bf2f(g[i * cy + c]) - scale * float(f[c * n + i]));
+    }
+}
+
 // Input features, channels-last [n][16] bf16:
 //  0-2 one-hot of bulk / next to an object / object,
 //  3-7 anchoring tensor of object sites: Q^B / S0 for homeotropic objects,
@@ -572,6 +585,27 @@ int qm_ws_mse_grad(const float* y, int cy, const float* target, const void* type
                        target, static_cast<const int8_t*>(types), n, scale, static_cast<uint16_t*>(g), lossPart,
                        off5(offset5));
     HIP_TRY(hipGetLastError());
+    return QM_OK;
+}
+
+// The energy term of the warm-start loss: the lattice holds the prediction
+// (qm_ws_apply); its forces are computed and -scale F added to the first five
+// channels of g at liquid-crystal sites.  The caller's stream is drained first.
+int qm_ws_energy_grad(qm_ldg* f, qm_lattice* L, float scale, void* g, int cy, void* stream) {
+    if (!f || !L || !g || cy < 5) return fail(QM_EINVAL, "bad argument");
+    HIP_TRY(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+    if (int rc = qm_ldg_compute_forces(f, L)) return rc;
+    const int blocks = grid_for(L->n);
+    if (L->precision == QM_FP64)
+        hipLaunchKernelGGL(energy_grad_kernel<double>, dim3(blocks), dim3(kWsThreads), 0, L->stream,
+                           state_set<double>(L, L->hCtrl.cur).f, L->dTypes, L->n, cy, scale,
+                           static_cast<uint16_t*>(g));
+    else
+        hipLaunchKernelGGL(energy_grad_kernel<float>, dim3(blocks), dim3(kWsThreads), 0, L->stream,
+                           state_set<float>(L, L->hCtrl.cur).f, L->dTypes, L->n, cy, scale,
+                           static_cast<uint16_t*>(g));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(L->stream));
     return QM_OK;
 }
 

@@ -12,18 +12,27 @@
 // Differences a caller can see:
 //  * one rank only: halos, rank topologies > 1x1x1 and GPU selection by MPI
 //    rank throw;
-//  * host views of Q, v, F are std::vector<double> in the reference's AoS
-//    order (N x {Qxx,Qxy,Qxz,Qyy,Qyz}) instead of GPUArray<dVec>;
+//  * GPUArray<dVec> / ArrayHandle keep the reference's host view (AoS dVec, one
+//    per site, {Qxx,Qxy,Qxz,Qyy,Qyz}); the arrays a model returns are views of
+//    its device state (structure of arrays), copied on a host handle's
+//    acquisition and written back on its release; device handles throw, as the
+//    device layout is not an array of dVec;
+//  * force::computeForces(GPUArray<dVec>&, zeroOutForce, type) computes every
+//    site's force at type 0; type 1 (the CPU path's second pass over boundary
+//    sites) adds nothing, as the device passes cover all sites together;
 //  * setCPUOperation(true) selects the reference CPU path's metric semantics,
 //    (false) the GPU path's; the arithmetic always runs on the GPU.
 #pragma once
 
+#include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <iostream>
 #include <memory>
 #include <random>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "qmin.h"
@@ -35,6 +44,9 @@ struct scalar3 {
 };
 struct scalar4 {
     double x, y, z, w;
+};
+struct int3 {
+    int x, y, z;
 };
 inline scalar3 make_scalar3(double x, double y, double z) { return scalar3{x, y, z}; }
 
@@ -129,13 +141,184 @@ class logSpacedIntegers {
     scalar base;
 };
 
+// profiler (src/utilities/profiler.h): wall-clock time of start()/end()
+// brackets, accumulated over calls
+class profiler {
+   public:
+    explicit profiler(std::string profilerName) : name(std::move(profilerName)) {}
+    void start() { startTime = std::chrono::high_resolution_clock::now(); }
+    void end() {
+        timeTaken += std::chrono::duration<double>(std::chrono::high_resolution_clock::now() - startTime).count();
+        functionCalls += 1;
+    }
+    double timing() { return functionCalls > 0 ? timeTaken / functionCalls : 0.0; }
+    void print() {
+        std::cout << "profiler \"" << name << "\": " << functionCalls << " calls, " << timing() << " s per call, "
+                  << timeTaken << " s in all" << std::endl;
+    }
+    std::string name;
+    int functionCalls = 0;
+    double timeTaken = 0.0;
+    std::chrono::time_point<std::chrono::high_resolution_clock> startTime;
+};
+
+// dVec (src/utilities/dDimensionalVectorTypes.h) with DIMENSION = 5: one
+// Q tensor {Qxx, Qxy, Qxz, Qyy, Qyz}
+struct dVec {
+    double x[5];
+    dVec() : x{0, 0, 0, 0, 0} {}
+    explicit dVec(double v) : x{v, v, v, v, v} {}
+    double& operator[](int i) { return x[i]; }
+    const double& operator[](int i) const { return x[i]; }
+    dVec& operator+=(const dVec& o) {
+        for (int k = 0; k < 5; ++k) x[k] += o.x[k];
+        return *this;
+    }
+    dVec& operator-=(const dVec& o) {
+        for (int k = 0; k < 5; ++k) x[k] -= o.x[k];
+        return *this;
+    }
+};
+inline dVec make_dVec(double v) { return dVec(v); }
+static_assert(sizeof(dVec) == 5 * sizeof(double), "dVec must be five packed doubles (AoS host layout)");
+
+// GPUArray / ArrayHandle (src/utilities/gpuarray.h:7-124)
+struct access_location {
+    enum Enum { host, device };
+};
+struct access_mode {
+    enum Enum { read, readwrite, overwrite };
+};
+
+template <class T>
+class ArrayHandle;
+
+// A host array of T.  Arrays returned by a model are views of one of its
+// device fields (Q, v, F: T = dVec; site types: T = int, read only): a host
+// ArrayHandle copies the field in (unless it overwrites) and, unless it only
+// reads, writes it back when it is destroyed.
+template <class T>
+class GPUArray {
+   public:
+    GPUArray() = default;
+    explicit GPUArray(unsigned int num_elements, bool = false) : host_(num_elements) {}
+    virtual ~GPUArray() = default;
+    unsigned int getNumElements() { return size(); }
+    unsigned int size() {
+        return bound() ? (unsigned int)nSites() : (unsigned int)host_.size();
+    }
+    virtual void resize(unsigned int num_elements) {
+        if (bound()) throw qminError(QM_EINVAL, "GPUArray: a model's array has the model's size");
+        host_.resize(num_elements);
+    }
+    void swap(GPUArray& other) {
+        if (bound() || other.bound()) throw qminError(QM_EINVAL, "GPUArray: a model's array cannot be swapped");
+        host_.swap(other.host_);
+    }
+    // binding to a lattice field (QM_FIELD_Q/V/F, or kTypes)
+    static constexpr int kTypes = 3;
+    void bind(qm_lattice* lat, int field) {
+        lat_ = lat;
+        field_ = field;
+    }
+    bool bound() const { return lat_ != nullptr; }
+    qm_lattice* lattice() const { return lat_; }
+    int field() const { return field_; }
+
+   private:
+    friend class ArrayHandle<T>;
+    long long nSites() const {
+        int lx = 0, ly = 0, lz = 0, prec = 0;
+        qm_lattice_dims(lat_, &lx, &ly, &lz, &prec);
+        return (long long)lx * ly * lz;
+    }
+    T* acquire(access_mode::Enum mode);
+    void release(access_mode::Enum mode);
+    std::vector<T> host_;
+    qm_lattice* lat_ = nullptr;
+    int field_ = -1;
+};
+
+template <class T>
+class ArrayHandle {
+   public:
+    ArrayHandle(GPUArray<T>& gpu_array, access_location::Enum location = access_location::host,
+                access_mode::Enum mode = access_mode::readwrite)
+        : arr_(gpu_array), mode_(mode) {
+        if (location == access_location::device)
+            throw qminError(QM_EINVAL, "ArrayHandle: device access is not offered (device storage is structure of "
+                                       "arrays, not an array of dVec)");
+        data = arr_.acquire(mode);
+    }
+    ~ArrayHandle() {
+        try {
+            arr_.release(mode_);
+        } catch (...) {
+        }
+    }
+    ArrayHandle(const ArrayHandle&) = delete;
+    ArrayHandle& operator=(const ArrayHandle&) = delete;
+    T* data;
+
+   private:
+    GPUArray<T>& arr_;
+    access_mode::Enum mode_;
+};
+
 // ---------------------------------------------------------------- models
 class simpleModel {
    public:
     virtual ~simpleModel() = default;
     virtual int getNumberOfParticles() = 0;
     virtual qm_lattice* qmLattice() = 0;
+    // simpleModel::returnPositions/returnForces/returnTypes/returnVelocities
+    // (simpleModel.h:63-73): views of the device state
+    GPUArray<dVec>& returnPositions() { return bound(positions, QM_FIELD_Q); }
+    GPUArray<dVec>& returnVelocities() { return bound(velocities, QM_FIELD_V); }
+    GPUArray<dVec>& returnForces() { return bound(forces, QM_FIELD_F); }
+    GPUArray<int>& returnTypes() { return bound(types, GPUArray<int>::kTypes); }
+    // simpleModel::getClassSize (simpleModel.h:95): GB of data the model holds
+    // (here its device state: two sets of Q, v, F, the site types and classes)
+    virtual scalar getClassSize() {
+        int lx = 0, ly = 0, lz = 0, prec = 0;
+        qm_lattice_dims(qmLattice(), &lx, &ly, &lz, &prec);
+        const double n = double(lx) * ly * lz;
+        return 1e-9 * (30.0 * n * (prec == QM_FP64 ? 8 : 4) + 2.0 * n);
+    }
+
+   private:
+    template <class T>
+    GPUArray<T>& bound(GPUArray<T>& a, int field) {
+        if (!a.bound()) a.bind(qmLattice(), field);
+        return a;
+    }
+    GPUArray<dVec> positions, velocities, forces;
+    GPUArray<int> types;
 };
+
+template <class T>
+T* GPUArray<T>::acquire(access_mode::Enum mode) {
+    if (!bound()) return host_.data();
+    host_.resize((size_t)nSites());
+    if (mode == access_mode::overwrite) return host_.data();
+    if constexpr (std::is_same<T, dVec>::value) {
+        qmin_host::check(qm_lattice_get_field(lat_, field_, reinterpret_cast<double*>(host_.data())), "ArrayHandle");
+    } else {
+        static_assert(std::is_same<T, int>::value, "model arrays are dVec or int");
+        qmin_host::check(qm_lattice_get_types(lat_, reinterpret_cast<int32_t*>(host_.data())), "ArrayHandle");
+    }
+    return host_.data();
+}
+
+template <class T>
+void GPUArray<T>::release(access_mode::Enum mode) {
+    if (!bound() || mode == access_mode::read) return;
+    // site types are read only through a handle: they change through objects
+    // (createWall, createSphericalColloid, createBoundaryFromFile)
+    if constexpr (std::is_same<T, dVec>::value)
+        qmin_host::check(qm_lattice_set_field(lat_, field_, reinterpret_cast<const double*>(host_.data())),
+                         "ArrayHandle");
+}
 
 class qTensorLatticeModel : public simpleModel {
    public:
@@ -309,6 +492,11 @@ class landauDeGennesLC {
         qmin_host::check(qm_lattice_load_hfield(lattice(), fn.c_str()), "setSpatiallyVaryingField");
         qmin_host::check(qm_ldg_set_varying_hfield(f_, 1, chi, mu0, deltaChi), "setSpatiallyVaryingField");
     }
+    // the reference's signature (landauDeGennesLC.h:93); rank (0,0,0) on one rank
+    void setSpatiallyVaryingField(std::string fname, scalar chi, scalar mu0, scalar deltaChi, int3 rankParity) {
+        (void)rankParity;
+        setSpatiallyVaryingField(fname, chi, mu0, deltaChi);
+    }
     // QM_SEMANTICS_REF_CPU or QM_SEMANTICS_REF_GPU
     void setSemantics(int s) { qmin_host::check(qm_ldg_set_semantics(f_, s), "setSemantics"); }
     void setModel(std::shared_ptr<cubicLattice> m) { model = m; }
@@ -317,6 +505,32 @@ class landauDeGennesLC {
     void computeForces() {
         qmin_host::check(qm_ldg_compute_forces(f_, lattice()), "computeForces");
     }
+    // force::computeForces(GPUArray<dVec>&, zeroOutForce, type) (baseForce.h:23,
+    // landauDeGennesLC.cpp:125-135): the forces of every site into `forces`
+    // (replacing its contents, or added to them), metric-corrected.  type 1 (the
+    // CPU path's boundary pass, which adds the boundary sites' terms) adds
+    // nothing here: the type-0 call has computed every site.
+    void computeForces(GPUArray<dVec>& forces, bool zeroOutForce = true, int type = 0) {
+        if (type != 0) return;
+        const bool ownField = forces.bound() && forces.lattice() == lattice() && forces.field() == QM_FIELD_F;
+        const size_t n = (size_t)model->getNumberOfParticles();
+        std::vector<dVec> before;
+        if (!zeroOutForce) {
+            ArrayHandle<dVec> h(forces, access_location::host, access_mode::read);
+            before.assign(h.data, h.data + n);
+        }
+        computeForces();
+        if (ownField && zeroOutForce) return;
+        std::vector<double> f(5 * n);
+        qmin_host::check(qm_lattice_get_field(lattice(), QM_FIELD_F, f.data()), "computeForces");
+        if (!forces.bound() && forces.getNumElements() < n) forces.resize((unsigned int)n);
+        ArrayHandle<dVec> h(forces, access_location::host, access_mode::overwrite);
+        for (size_t i = 0; i < n; ++i)
+            for (int k = 0; k < 5; ++k) h.data[i][k] = (zeroOutForce ? 0.0 : before[i][k]) + f[5 * i + k];
+    }
+    // landauDeGennesLC::getClassSize (landauDeGennesLC.h:102-106): GB of data the
+    // force holds (its energy components and density on the host)
+    scalar getClassSize() { return 1e-9 * sizeof(scalar) * (energyComponents.size() + energyDensity.size() + 20); }
     // landauDeGennesLC::computeEnergy: the total (not per site); fills
     // energyComponents {phase, distortion, anchoring, E, H} and, as the
     // reference does, the per-site energyDensity (n doubles copied to the host
@@ -381,6 +595,9 @@ class updater {
     void setMaximumIterations(int maxIt) { maxIterations = maxIt; }
     virtual scalar getMaxForce() { return 0.0; }
     virtual void performUpdate() = 0;
+    // updater::getClassSize (baseUpdater.h:82-85): GB of data the updater holds
+    // on the host (its device history belongs to the lattice's allocation)
+    virtual scalar getClassSize() { return 1e-9 * (6 * sizeof(int) + 2 * sizeof(bool) + sizeof(scalar)); }
     void Update(int) { performUpdate(); }
     int iterations = 0;
     int maxIterations = 1000;
@@ -623,6 +840,10 @@ class multirankSimulation {
 
     int integerTimestep = 0;
     long long NActive = 0;
+    // multirankSimulation.h:57,131: communication time (none on one rank) and
+    // the rank's parity in the rank grid
+    profiler p1 = profiler("total communication time");
+    int3 rankParity{0, 0, 0};
 
    private:
     qm_lattice* lat() {

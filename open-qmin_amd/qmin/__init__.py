@@ -55,6 +55,7 @@ _API = {
     "qm_ws_weights_to_bf16": [_VP, ctypes.c_longlong, _VP, _VP, _i, _i, _VP],
     "qm_ws_mse_grad": [_VP, _i, _VP, _VP, ctypes.c_longlong, ctypes.c_float, _D, _VP, _VP, _i, _VP],
     "qm_ws_features": [_VP, _d, _d, _d, _VP],
+    "qm_ws_energy_grad": [_VP, _VP, ctypes.c_float, _VP, _i, _VP],
     "qm_ws_types": [_VP, _VP],
     "qm_ws_apply": [_VP, _VP, _i, _D],
     "qm_ws_extract_q": [_VP, _VP],

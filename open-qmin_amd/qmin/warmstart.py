@@ -78,11 +78,12 @@ class Sample:
     """Device-resident training pair: features, converged Q, site types, and the
     uniform far-field Q the network's output corrects."""
 
-    def __init__(self, dims, feats, target, types, offset):
+    def __init__(self, dims, feats, target, types, offset, lattice=None):
         self.dims, self.feats, self.target, self.types = dims, feats, target, types
         self.offset = np.ascontiguousarray(offset, np.float64)
         self.n = dims[0] * dims[1] * dims[2]
         self.n_lc = int((types.cpu().numpy() <= 0).sum()) if hasattr(types, "cpu") else 0
+        self.lattice = lattice   # the lattice with this geometry (for the energy term), or None
 
 
 def lattice_features(lat, director):
@@ -208,20 +209,34 @@ class WarmStartNet:
                                        _dp(sample.offset), _ptr(g), _ptr(part), blocks, _stream()))
         return g, part
 
-    def train_step(self, sample, lr=1e-3):
+    def energy_grad(self, y, sample, force, weight):
+        """Adds the energy term of the loss, weight * E(prediction) / (5 n_lc):
+        the prediction goes into the sample's lattice, whose force F (minus the
+        energy gradient, shaped by the metric) gives -weight F / (5 n_lc)."""
+        _check(self.lib.qm_ws_apply(sample.lattice.h, _ptr(y), 16, _dp(sample.offset)))
+        return weight / max(sample.n_lc * 5, 1)
+
+    def train_step(self, sample, lr=1e-3, force=None, energy_weight=0.0):
         acts = self.forward(sample.feats, sample.dims)
         g, part = self.loss_and_grad(acts[-1], sample)
+        if energy_weight > 0 and force is not None and sample.lattice is not None:
+            scale = self.energy_grad(acts[-1], sample, force, energy_weight)
+            _check(self.lib.qm_ws_energy_grad(force.h, sample.lattice.h, scale, _ptr(g), 16, _stream()))
         self.backward(acts, g, sample.dims)
         self.adam_step(lr)
         return part
 
-    def train(self, samples, epochs=4, lr=1e-3, seed=0, log=None):
+    def train(self, samples, epochs=4, lr=1e-3, seed=0, log=None, force=None, energy_weight=0.0):
+        """Adam over the samples; the loss is the mean squared error to the
+        converged Q plus, with energy_weight > 0 and samples that keep their
+        lattice, energy_weight times the Landau-de Gennes energy of the
+        prediction per liquid-crystal component (force: the force computer)."""
         torch = _torch()
         rng = np.random.default_rng(seed)
         history = []
         for ep in range(epochs):
             order = rng.permutation(len(samples))
-            parts = [self.train_step(samples[k], lr) for k in order]
+            parts = [self.train_step(samples[k], lr, force, energy_weight) for k in order]
             torch.cuda.synchronize()
             mse = sum(float(p.sum().item()) / max(samples[k].n_lc * 5, 1) for p, k in zip(parts, order)) / len(order)
             history.append(mse)
@@ -282,9 +297,10 @@ def build_lattice(qmin, spec, precision, S0):
     return lat
 
 
-def make_dataset(qmin, specs, force, fc=1e-7, max_iter=4000, batch=128, log=None):
+def make_dataset(qmin, specs, force, fc=1e-7, max_iter=4000, batch=128, log=None, keep_lattices=False):
     """Minimize the geometries with the ensemble minimizer (FP32) and keep
-    device-resident training samples."""
+    device-resident training samples (with their lattices, for the energy term
+    of the loss, when keep_lattices)."""
     A, B, C, S0 = workloads.phase_constants()
     samples = []
     for start in range(0, len(specs), batch):
@@ -296,7 +312,8 @@ def make_dataset(qmin, specs, force, fc=1e-7, max_iter=4000, batch=128, log=None
             q = lat.q()
             types = lat.types()
             d = far_field_director(q, types)
-            s = Sample(lat.shape, lattice_features(lat, d), lattice_q(lat), lattice_types(lat), uniform_q(d, S0))
+            s = Sample(lat.shape, lattice_features(lat, d), lattice_q(lat), lattice_types(lat), uniform_q(d, S0),
+                       lat if keep_lattices else None)
             samples.append(s)
         if log:
             log("minimized %d training lattices in %.1f s (iterations mean %.0f, converged %d)"

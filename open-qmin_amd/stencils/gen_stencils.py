@@ -21,6 +21,10 @@ differences into one linear operator on the 19-point neighbourhood (centre,
 
   ldg_fast_lin_a / _b(W, q, sm, sc, sp, h, acc)   device: acc[r] += W[k] Q_c(p)
   ../csrc/gen/ldg_fast_host.inc                   host: C1..C4 per entry k
+  ldg_ring_lin_a / _b(W, q, sm, sc, sp, h, cm, cc, cp, acc)
+                                      device: the same operator over rings of
+                                      single planes, with the operator's few
+                                      distinct weights (host: kPlaneLin*)
 variant a covers L1..L3 (L4 = 0), variant b L1..L4.  The bulk L6 form, a
 quadratic form in the same central differences, becomes one over the 19 point
 values, generated together with variant b so that each value is read once:
@@ -181,7 +185,66 @@ def gen_fast(tables):
                 last = (p, c)
             dev.append("    acc[%d] += W[%d] * v;" % (r, idx))
         dev.append("}")
+        rows = [tuple(float(t.get(k, 0)) for t in terms) for k in keys]
+        distinct = distinct_weights(rows)
+        dev += gen_direct_distinct(name, keys, rows, distinct)
+        host.append("// distinct weights of ldg_ring_lin_%s (each entry is one of them or its negative)" % name)
+        host.append("static const int kPlaneLin%sCount = %d;" % (name.upper(), len(distinct)))
+        host.append("static const double kPlaneLin%s[%d][4] = {" % (name.upper(), len(distinct)))
+        for row in distinct:
+            host.append("    {%s}," % ", ".join(repr(v) for v in row))
+        host.append("};")
     return dev, host, keys
+
+
+def distinct_weights(rows):
+    """The distinct coefficient rows of an operator up to sign (first nonzero
+    coefficient positive).  Negating a row negates its L-weighted sum exactly,
+    so an entry may use W[j] or -W[j] in place of its own weight."""
+    distinct = []
+    for row in rows:
+        lead = next(v for v in row if v != 0)
+        canon = row if lead > 0 else tuple(-v for v in row)
+        if canon not in distinct:
+            distinct.append(canon)
+    return distinct
+
+
+def gen_direct_distinct(name, keys, rows, distinct):
+    """ldg_fast_lin_<name> over a ring of three planes of single-plane rows
+    (q[slot][c][NP], row stride HXS), with the distinct weights (a table per
+    variant, emitted for the host); the site's own column (the points (0, 0, -1), (0, 0, 0)
+    and (0, 0, 1)) comes from registers cm, cc, cp instead of the ring:
+    ldg_ring_lin_<name>(W, q, sm, sc, sp, h, cm, cc, cp, acc)."""
+    lines = ["template <int HXS, int NP, typename R, typename WP>",
+             "__device__ __forceinline__ void ldg_ring_lin_%s(WP W, const R (*q)[5][NP], int sm, int sc, int sp,"
+             " int h, const R* cm, const R* cc, const R* cp, R* acc) {" % name,
+             "    R v;"]
+    own = {(0, 0, -1): "cm", (0, 0, 0): "cc", (0, 0, 1): "cp"}
+    last = None
+    points_seen = []
+    for (r, p, c), row in zip(keys, rows):
+        if p not in points_seen:
+            if points_seen and len(points_seen) % 2 == 0:
+                lines.append("    __builtin_amdgcn_sched_barrier(0);")
+            points_seen.append(p)
+        if (p, c) != last:
+            if p in own:
+                lines.append("    v = %s[%d];" % (own[p], c))
+            else:
+                slot = {-1: "sm", 0: "sc", 1: "sp"}[p[2]]
+                off = "h"
+                if p[0]:
+                    off += " %s 1" % ("+" if p[0] > 0 else "-")
+                if p[1]:
+                    off += " %s HXS" % ("+" if p[1] > 0 else "-")
+                lines.append("    v = q[%s][%d][%s];" % (slot, c, off))
+            last = (p, c)
+        lead = next(v for v in row if v != 0)
+        canon = row if lead > 0 else tuple(-v for v in row)
+        lines.append("    acc[%d] %s= W[%d] * v;" % (r, "+" if lead > 0 else "-", distinct.index(canon)))
+    lines.append("}")
+    return lines
 
 
 L6_FACES = ((-1, 0, 0), (1, 0, 0), (0, -1, 0), (0, 1, 0), (0, 0, -1), (0, 0, 1))

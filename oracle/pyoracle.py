@@ -67,7 +67,7 @@ def oracle_lib():
         _sig(lib, "orc_neighbors", None, [_VP, _I])
         _sig(lib, "orc_get_types", None, [_VP, _I])
         _sig(lib, "orc_set_types", None, [_VP, _I])
-        for n in ("orc_get_q", "orc_set_q", "orc_get_v", "orc_get_f"):
+        for n in ("orc_get_q", "orc_set_q", "orc_get_v", "orc_set_v", "orc_get_f"):
             _sig(lib, n, None, [_VP, _D])
         _sig(lib, "orc_random_init", None, [_VP, _i, _d])
         _sig(lib, "orc_random_uniform_init", None, [_VP, _i, _d])
@@ -224,6 +224,10 @@ class OracleSystem:
     def set_q(self, q):
         q = np.ascontiguousarray(q, np.float64)
         self.lib.orc_set_q(self.h, dptr(q))
+
+    def set_v(self, q):
+        q = np.ascontiguousarray(q, np.float64)
+        self.lib.orc_set_v(self.h, dptr(q))
 
     def random_init(self, seed, S0):
         self.lib.orc_random_init(self.h, seed, S0)

@@ -561,6 +561,7 @@ void orc_set_types(orc_system* s, const int32_t* in) { std::memcpy(s->types.data
 void orc_get_q(const orc_system* s, double* out) { std::memcpy(out, s->q.data(), 5 * s->n * sizeof(double)); }
 void orc_set_q(orc_system* s, const double* in) { std::memcpy(s->q.data(), in, 5 * s->n * sizeof(double)); }
 void orc_get_v(const orc_system* s, double* out) { std::memcpy(out, s->v.data(), 5 * s->n * sizeof(double)); }
+void orc_set_v(orc_system* s, const double* in) { std::memcpy(s->v.data(), in, 5 * s->n * sizeof(double)); }
 void orc_get_f(const orc_system* s, double* out) { std::memcpy(out, s->f.data(), 5 * s->n * sizeof(double)); }
 int orc_n_objects(const orc_system* s) { return (int)s->objects.size(); }
 

@@ -81,6 +81,7 @@ double orc_energy_cb(void* ctx, int n, const double* q);
 int orc_fire(orc_system* s, double dt, double alphaStart, double dtMax, double dtInc, double dtDec,
              double alphaDec, int nMin, double fc, double alphaMin, int maxIter, double* trace);
 void orc_get_v(const orc_system* s, double* out);
+void orc_set_v(orc_system* s, const double* in);
 void orc_get_f(const orc_system* s, double* out);
 int orc_adam(orc_system* s, double b1, double b2, double eps, double dt, double fc, int maxIter, double* trace);
 int orc_nesterov(orc_system* s, double dt, double mu, double fc, int scheduled, int maxIter, double* trace);

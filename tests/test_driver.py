@@ -172,3 +172,36 @@ def test_driver_save_schedules(tmp_path, schedule):
     assert np.all(rows[:, :3] % 2 == 0)
     q = o.q().reshape(12, 12, 12, 5)[::2, ::2, ::2].reshape(-1, 5)
     np.testing.assert_allclose(rows[:, 3:8], q, rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.gpu
+def test_reference_array_views_and_force_into_array():
+    """open-qmin_amd/host/array_views_example.cpp: reference-style code that
+    edits Q through a read-write ArrayHandle on returnPositions(), reads the
+    types through returnTypes(), and calls computeForces(GPUArray<dVec>&, zero,
+    type) into the model's array and into its own; forces against the oracle
+    on the same state, the added call doubling them exactly, a device handle
+    refused."""
+    exe = os.path.join(REPO, "open-qmin_amd", "lib", "array-views-example")
+    p = subprocess.run([exe, "12", "77"], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    out = {}
+    for line in p.stdout.splitlines():
+        key, *vals = line.split()
+        out.setdefault(key, []).append(vals)
+    o = po.OracleSystem(12, 12, 12)
+    o.random_init(77, S0)
+    o.sphere((6.0, 6.0, 6.0), 3.0, 0, 5.3, S0)
+    q = o.q()
+    q[1, 0] += 0.125
+    q[1, 4] -= 0.0625
+    o.set_q(q)
+    assert [float(v) for v in out["q1"][0]] == [q[1, 0], q[1, 4]]
+    assert int(out["objects"][0][0]) == int((o.types() > 0).sum())
+    assert float(out["doubled"][0][0]) == 0.0
+    o.set_params(A, B, C, 2.32, 0.5, 0.3, multi=True, semantics=1)
+    ref = o.forces()
+    for vals in out["f"]:
+        i = int(vals[0])
+        np.testing.assert_allclose([float(v) for v in vals[1:]], ref[i], rtol=1e-12, atol=1e-13)
+    assert out["device-handle-throws"][0][0] == "1"
