@@ -213,6 +213,15 @@ int qm_fire_set_force_cutoff(qm_fire* m, double forceCutoff);
 int qm_fire_minimize(qm_fire* m, qm_ldg* f, qm_lattice* lat);
 /* run exactly n iterations (no tolerance test): throughput measurement */
 int qm_fire_run_iterations(qm_fire* m, qm_ldg* f, qm_lattice* lat, int n);
+/* qm_fire_run_iterations without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The passes'
+ * work-group partial sums go to a fixed 1.5 MiB array in the library's device
+ * image (65536 slots, one per work-group of a pass: QM_EINVAL on a lattice
+ * that needs more), one copy per GPU, shared by every call on that GPU: calls
+ * that overlap on one GPU race.  No trace is recorded.  Results equal
+ * qm_fire_run_iterations'. */
+int qm_fire_run_iterations_noalloc(qm_fire* m, qm_ldg* f, qm_lattice* lat, int n);
 int qm_fire_get_state(const qm_fire* m, qm_fire_state* out);
 /* per-iteration trace {forceMax, power, alpha, deltaT, scaling, nPositive}: capacity in
  * iterations (0 disables); qm_fire_get_trace copies up to cap rows, returns the count in *n */

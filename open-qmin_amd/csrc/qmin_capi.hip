@@ -9,6 +9,7 @@
 #include <cstring>
 #include <fstream>
 #include <memory>
+#include <mutex>
 #include <random>
 #include <sstream>
 #include <string>
@@ -231,6 +232,7 @@ int qm_lattice_create(int lx, int ly, int lz, int precision, qm_lattice** out) {
     L->precision = precision;
     L->realSize = precision == QM_FP64 ? 8 : 4;
     L->types.assign(n, 0);
+    HIP_TRY(hipGetDevice(&L->device));
     HIP_TRY(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking));
     HIP_TRY(hipMalloc(&L->dTypes, n));
     HIP_TRY(hipMalloc(&L->dClass, n));
@@ -984,13 +986,19 @@ static int march_blocks(const qm_lattice* L, int mode) {
 }
 
 // One pass (force only or one FIRE iteration); kernels only, no host sync.
+// partials: the work-group partial sums (null: the lattice's own buffer)
 template <typename R>
-static int run_pass(qm_lattice* L, const qm_ldg* f, int mode, const FireConst& fire, bool gated = false) {
+static int run_pass(qm_lattice* L, const qm_ldg* f, int mode, const FireConst& fire, bool gated = false,
+                    double* partials = nullptr) {
     choose_zchunk(L, mode, terms_of(f));
     MarchArgs<R> ma{};
     BoundaryArgs<R> ba{};
     const int tile = active_tile(L, mode);
     fill_args<R>(L, f, ma, ba, fire, tile);
+    if (partials) {
+        ma.partials = partials;
+        ba.partials = partials;
+    }
     ma.gate = gated ? 1 : 0;
     ba.gate = ma.gate;
     const int terms = terms_of(f);
@@ -1354,16 +1362,74 @@ FireConst qmin::fire_const(const qm_fire* m, const qm_lattice* L) {
     return fp;
 }
 
-// Runs iterations on the device until the control block reports done, syncing
-// with the host once per batch.
-static int fire_drive(qm_fire* m, qm_ldg* f, qm_lattice* L, int maxIter, bool tolerance) {
-    if (int rc = prepare(L, f)) return rc;
-    if (m->traceCap > 0 && L->traceCap < m->traceCap) {
-        if (L->dTrace) HIP_TRY(hipFree(L->dTrace));
-        HIP_TRY(hipMalloc(&L->dTrace, 6 * (size_t)m->traceCap * sizeof(double)));
-        L->traceCap = m->traceCap;
+// ------------------------------------------------------------------ allocation-free FIRE
+// Work-group partial sums of qm_fire_run_iterations_noalloc: a fixed array in
+// the library's device image, so that the call needs no hipMalloc.  Each GPU the
+// library runs on holds its own copy until the process ends.  Every slot a pass
+// reads is written by that pass first, so nothing has to start at zero; calls
+// that overlap on one GPU share the array and race.
+constexpr long long kScratchSlots = 65536;   // partial-sum slots of 3 doubles: 1.5 MiB
+constexpr int kScratchDevices = 64;
+static __device__ __align__(256) double scratch[3 * kScratchSlots];
+static std::mutex g_scratchMutex;
+static double* g_scratchPtr[kScratchDevices] = {};
+
+// Device address of `scratch` on the lattice's GPU, looked up once per device
+// with that device current; host code never uses the array's own name.
+static int scratch_for(const qm_lattice* L, double** out) {
+    if (L->device < 0 || L->device >= kScratchDevices) return fail(QM_EINVAL, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    if (!g_scratchPtr[L->device]) {
+        int prev = 0;
+        HIP_TRY(hipGetDevice(&prev));
+        HIP_TRY(hipSetDevice(L->device));
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        const hipError_t r = hipSetDevice(prev);
+        if (e != hipSuccess) return fail(QM_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        if (r != hipSuccess) return fail(QM_EHIP, std::string("hipSetDevice: ") + hipGetErrorString(r));
+        g_scratchPtr[L->device] = static_cast<double*>(p);
     }
-    const int traceCap = m->traceCap;
+    *out = g_scratchPtr[L->device];
+    return QM_OK;
+}
+
+// What prepare() does, minus every allocation (the site types are on the
+// device already: qm_lattice_create and every object call upload them at
+// once), provided the passes' partial sums fit the scratch array.
+static int prepare_noalloc(qm_lattice* L, const qm_ldg* f, double** partials) {
+    if (!L || !f) return fail(QM_EINVAL, "null handle");
+    if (f->hvarOn && !L->dHField)
+        return fail(QM_ESTATE, "the force uses a spatially varying field but the lattice has none");
+    L->terms = terms_of(f);
+    long long need = 0;
+    for (int mode : {kForce, kFire}) {
+        choose_zchunk(L, mode, L->terms);
+        need = std::max(need, (long long)march_blocks(L, mode) + boundary_blocks(L->nBoundary));
+    }
+    if (need > kScratchSlots)
+        return fail(QM_EINVAL, "the lattice needs " + std::to_string(need) + " partial-sum slots; the "
+                               "allocation-free call holds at most " + std::to_string(kScratchSlots));
+    if (int rc = upload_weights(L, f)) return rc;
+    return scratch_for(L, partials);
+}
+
+// Runs iterations on the device until the control block reports done, syncing
+// with the host once per batch.  With noalloc the partial sums go to the scratch
+// array and no trace is kept.
+static int fire_drive(qm_fire* m, qm_ldg* f, qm_lattice* L, int maxIter, bool tolerance, bool noalloc = false) {
+    double* partials = nullptr;
+    if (noalloc) {
+        if (int rc = prepare_noalloc(L, f, &partials)) return rc;
+    } else {
+        if (int rc = prepare(L, f)) return rc;
+        if (m->traceCap > 0 && L->traceCap < m->traceCap) {
+            if (L->dTrace) HIP_TRY(hipFree(L->dTrace));
+            HIP_TRY(hipMalloc(&L->dTrace, 6 * (size_t)m->traceCap * sizeof(double)));
+            L->traceCap = m->traceCap;
+        }
+    }
+    const int traceCap = noalloc ? 0 : m->traceCap;
     if (int rc = sync_ctrl_to_host(L)) return rc;
     Ctrl& c = L->hCtrl;
     c.done = 0;
@@ -1381,16 +1447,16 @@ static int fire_drive(qm_fire* m, qm_ldg* f, qm_lattice* L, int maxIter, bool to
     FireConst fp = fire_const(m, L);
     if (!tolerance) fp.fc = -1.0;  // never converged: exactly maxIter - iterations steps
     // sim->computeForces() at the start of minimize (energyMinimizerFIRE.cpp:157)
-    int rc = L->precision == QM_FP64 ? run_pass<double>(L, f, kForce, fp)
-                                     : run_pass<float>(L, f, kForce, fp);
+    int rc = L->precision == QM_FP64 ? run_pass<double>(L, f, kForce, fp, false, partials)
+                                     : run_pass<float>(L, f, kForce, fp, false, partials);
     if (rc) return rc;
     const int batch = L->n >= (1 << 22) ? 16 : 64;
     while (true) {
         int todo = std::min(batch, maxIter - L->hCtrl.iterations);
         if (todo <= 0) todo = 1;  // at least one iteration per minimize call
         for (int k = 0; k < todo; ++k) {
-            rc = L->precision == QM_FP64 ? run_pass<double>(L, f, kFire, fp)
-                                         : run_pass<float>(L, f, kFire, fp);
+            rc = L->precision == QM_FP64 ? run_pass<double>(L, f, kFire, fp, false, partials)
+                                         : run_pass<float>(L, f, kFire, fp, false, partials);
             if (rc) return rc;
         }
         if ((rc = sync_ctrl_to_host(L))) return rc;
@@ -1415,8 +1481,8 @@ static int fire_drive(qm_fire* m, qm_ldg* f, qm_lattice* L, int maxIter, bool to
     // The pass that keeps only Q and v leaves F stale on sites it finishes
     // itself: recompute F of the final state, which the caller may read.
     if (is_stream_tile(active_tile(L, kFire))) {
-        rc = L->precision == QM_FP64 ? run_pass<double>(L, f, kForce, fp)
-                                     : run_pass<float>(L, f, kForce, fp);
+        rc = L->precision == QM_FP64 ? run_pass<double>(L, f, kForce, fp, false, partials)
+                                     : run_pass<float>(L, f, kForce, fp, false, partials);
         if (rc) return rc;
     }
     // energyMinimizerFIRE::fireStep mixes the velocity with the force of the
@@ -1437,6 +1503,11 @@ int qm_fire_minimize(qm_fire* m, qm_ldg* f, qm_lattice* L) {
 int qm_fire_run_iterations(qm_fire* m, qm_ldg* f, qm_lattice* L, int n) {
     if (!m || n < 1) return fail(QM_EINVAL, "bad argument");
     return fire_drive(m, f, L, m->iterations + n, false);
+}
+
+int qm_fire_run_iterations_noalloc(qm_fire* m, qm_ldg* f, qm_lattice* L, int n) {
+    if (!m || n < 1) return fail(QM_EINVAL, "bad argument");
+    return fire_drive(m, f, L, m->iterations + n, false, true);
 }
 
 // ------------------------------------------------------------------ timing

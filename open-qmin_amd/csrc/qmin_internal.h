@@ -27,6 +27,7 @@ struct qm_lattice {
     int lx = 0, ly = 0, lz = 0, precision = QM_FP64;
     long long n = 0;
     size_t realSize = 8;
+    int device = 0;                  // the GPU that holds the lattice (current at creation)
     hipStream_t stream = nullptr;
     std::vector<int32_t> types;      // host copy, reference encoding
     std::vector<HostObject> objects;

@@ -104,6 +104,7 @@ _API = {
     "qm_fire_set_force_cutoff": [_VP, _d],
     "qm_fire_minimize": [_VP, _VP, _VP],
     "qm_fire_run_iterations": [_VP, _VP, _VP, _i],
+    "qm_fire_run_iterations_noalloc": [_VP, _VP, _VP, _i],
     "qm_fire_get_state": [_VP, ctypes.POINTER(FireState)],
     "qm_fire_set_trace": [_VP, _i],
     "qm_fire_get_trace": [_VP, _D, _i, ctypes.POINTER(ctypes.c_int)],
@@ -454,6 +455,12 @@ class FIRE:
 
     def run_iterations(self, force, lattice, n):
         _check(self.lib.qm_fire_run_iterations(self.h, force.h, lattice.h, n))
+        return self.state()
+
+    def run_iterations_noalloc(self, force, lattice, n):
+        """run_iterations with no device allocation (partial sums in the
+        library's fixed scratch array; no trace)."""
+        _check(self.lib.qm_fire_run_iterations_noalloc(self.h, force.h, lattice.h, n))
         return self.state()
 
     def state(self):
