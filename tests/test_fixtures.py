@@ -1,12 +1,14 @@
 """The reference's own input files and the error paths of the C ABI.
 
-tests/golden/assets/ holds two boundary files that ship with the reference
-(assets/oneSphereExample.txt: one homeotropic sphere, W 5.3, S0 0.53, 4224
-sites; assets/twoSphereExample.txt: a homeotropic and a degenerate-planar
-sphere of 540 sites each), in the format multirankSimulation::
+tests/golden/assets/ holds three input files that ship with the reference: two
+boundary files (assets/oneSphereExample.txt: one homeotropic sphere, W 5.3,
+S0 0.53, 4224 sites; assets/twoSphereExample.txt: a homeotropic and a
+degenerate-planar sphere of 540 sites each), in the format multirankSimulation::
 createBoundaryFromFile reads (multirankSimulationBoundaries.cpp:41-96): a line
 with the number of objects, then per object a line "type W S0 count" and count
-lines "x y z Qxx Qxy Qxz Qyy Qyz".
+lines "x y z Qxx Qxy Qxz Qyy Qyz"; and my_init_file_x0y1z0.txt (gzipped), an
+initial configuration in the state-file format (one rank's share of a 50^3
+cholesteric helix).
 
 CPU: the files read by read_boundary_file below build the same site types
 in the oracle as in the reference's compiled cubicLattice.  GPU: the library's
@@ -138,6 +140,67 @@ def test_state_file_round_trip_and_reference_format(tmp_path):
     m.set_trace(16)
     m.minimize(f, g3)
     np.testing.assert_allclose(m.trace()[:, 0], tr_ref[:, 0], rtol=1e-10)
+
+
+def helix_q(z, lz=50, S=0.53):
+    """Q = (3S/2)(n n - I/3) of the cholesteric helix n = (cos 2 pi z/lz,
+    sin 2 pi z/lz, 0) that the reference's tools/initHelper.py wrote into
+    my_init_file_*.txt (pitch lz along z, S = 0.53)."""
+    n = np.stack([np.cos(2 * np.pi * z / lz), np.sin(2 * np.pi * z / lz), 0 * z], 1)
+    Q = 1.5 * S * (n[:, :, None] * n[:, None, :] - np.eye(3) / 3)
+    return np.stack([Q[:, 0, 0], Q[:, 0, 1], Q[:, 0, 2], Q[:, 1, 1], Q[:, 1, 2]], 1)
+
+
+def init_file_rows():
+    import gzip
+    with gzip.open(os.path.join(ASSETS, "my_init_file_x0y1z0.txt.gz"), "rt") as fh:
+        return np.loadtxt(fh)
+
+
+def test_init_file_fixture_is_the_helix():
+    """my_init_file_x0y1z0.txt (gzipped here) is rank (0,1,0) of a 2x2x1 split of
+    a 50^3 box: 25 x 25 x 50 sites in global coordinates (x 0..24, y 25..49),
+    type 0, the helix's Q to six decimals."""
+    rows = init_file_rows()
+    assert rows.shape == (31250, 10)
+    assert rows[:, 0].min() == 0 and rows[:, 0].max() == 24 and rows[:, 1].min() == 25 and rows[:, 1].max() == 49
+    assert np.all(rows[:, 8] == 0)
+    assert np.abs(rows[:, 3:8] - helix_q(rows[:, 2])).max() <= 5e-7
+
+
+@pytest.mark.gpu
+def test_load_state_of_the_reference_init_file(tmp_path):
+    """qm_lattice_load_state on the reference's initial-configuration file: the
+    sites it lists take its values (global coordinates, as loadState reads them
+    on one rank), every other site keeps its Q; FIRE from there follows the
+    oracle started from the same Q."""
+    import qmin
+    import gzip
+    rows = init_file_rows()
+    path = str(tmp_path / "my_init_file_x0y1z0.txt")
+    with gzip.open(os.path.join(ASSETS, "my_init_file_x0y1z0.txt.gz"), "rb") as src, open(path, "wb") as dst:
+        dst.write(src.read())
+    g = qmin.Lattice(50, 50, 50)
+    g.random_init(77, S0)
+    before = g.q()
+    g.load_state(path)
+    q = g.q()
+    site = (rows[:, 0] + 50 * (rows[:, 1] + 50 * rows[:, 2])).astype(int)
+    assert np.array_equal(q[site], rows[:, 3:8])
+    rest = np.ones(g.n, bool)
+    rest[site] = False
+    assert np.array_equal(q[rest], before[rest])
+    assert np.abs(q[site] - helix_q(rows[:, 2])).max() <= 5e-7
+    o = po.OracleSystem(50, 50, 50)
+    o.set_q(q)
+    o.set_params(A, B, C, 2.32, semantics=1)
+    f = qmin.LandauDeGennesLC(A, B, C, 2.32, semantics=1)
+    _, tr_ref = o.fire(max_iter=12)
+    m = qmin.FIRE(max_iter=12)
+    m.set_trace(16)
+    m.minimize(f, g)
+    np.testing.assert_allclose(m.trace()[:, 0], tr_ref[:, 0], rtol=1e-10)
+    np.testing.assert_allclose(g.q(), o.q(), rtol=0, atol=1e-11)
 
 
 @pytest.mark.gpu
