@@ -78,35 +78,61 @@ def gpu_throughput(qmin, L, precision, steps, warmup, zchunk=0):
                 forceMax=st.forceMax, iterations=st.iterations, n_boundary=lat.n_boundary())
 
 
-def time_to_tolerance(qmin, L, mode, fc=1e-8, switch_fc=None, max_iter=400000):
-    """FIRE from the seeded random start until forceMax < fc (the reference's
-    definition, energyMinimizerFIRE.cpp:110).  mode "fp64": all FP64; "mixed":
-    FP32 state and stencils with FP64 reductions until forceMax < switch_fc
-    (default fc), then the state is converted on the device and the same FIRE
-    minimizer (dt, alpha, N+ kept) polishes in FP64 down to fc."""
+# FP32 rounding floor of forceMax on the 256^3 workload: 2.2e-10 (a 30,000-
+# iteration FP32 trace levels off there).  The mixed runs aiming below it leave
+# FP32 at 3e-10, where the FP32 trajectory still descends at full speed.
+MIXED_SWITCH = 3e-10
+
+
+def tolerance_chain(qmin, L, mode, fcs, switch_fc=None, max_iter=400000, ring=False):
+    """FIRE from the seeded random start, stopping at each forceMax tolerance of
+    fcs in turn (the reference's definition, energyMinimizerFIRE.cpp:110): one
+    run, the minimizer continuing with its dt, alpha and N+ after each stop, so
+    each stage's time is what a run to that tolerance takes.  mode "fp64": all
+    FP64; "mixed": FP32 state and stencils with FP64 reductions until forceMax <
+    max(fcs[0], switch_fc), then the state is converted on the device and the
+    same minimizer polishes in FP64.  The energy (and with ring=True the defect
+    ring of the first stage) is taken outside the timed region."""
     prec = qmin.QM_FP64 if mode == "fp64" else qmin.QM_FP32
     lat, force = build_workload(qmin, L, prec)
-    first_fc = fc if mode == "fp64" else max(fc, switch_fc or fc)
+    first_fc = fcs[0] if mode == "fp64" else max(fcs[0], switch_fc or fcs[0])
     m = qmin.FIRE(fc=first_fc, max_iter=max_iter, **fire_params())
+    elapsed = 0.0
     t0 = time.perf_counter()
     st = m.minimize(force, lat)
     first = st.iterations
-    t1 = time.perf_counter()
+    fp32_seconds = time.perf_counter() - t0
+    elapsed += fp32_seconds
     if mode == "mixed":
+        t0 = time.perf_counter()
         lat.set_precision(qmin.QM_FP64)
-        m.set_force_cutoff(fc)
-        m.set_max_iterations(first + max_iter)
-        st = m.minimize(force, lat)
-    t2 = time.perf_counter()
-    e, _ = force.compute_energy(lat)
-    out = dict(seconds=t2 - t0, iterations=st.iterations, forceMax=st.forceMax, converged=bool(st.converged),
-               energy_per_site=e / lat.n_active())
-    if mode == "mixed":
-        out.update(fp32_until=first_fc, fp32_iterations=first, fp32_seconds=t1 - t0,
-                   polish_iterations=st.iterations - first, polish_seconds=t2 - t1)
-    print("time to forceMax<%g (%s): %d iterations, %.2f s" % (fc, mode, st.iterations, t2 - t0), file=sys.stderr,
-          flush=True)
+        elapsed += time.perf_counter() - t0
+    out = []
+    for k, fc in enumerate(fcs):
+        if mode == "mixed" or k > 0:
+            m.set_force_cutoff(fc)
+            m.set_max_iterations(m.state().iterations + max_iter)
+            t0 = time.perf_counter()
+            st = m.minimize(force, lat)
+            elapsed += time.perf_counter() - t0
+        e, _ = force.compute_energy(lat)
+        r = dict(forceMax_below=fc, seconds=elapsed, iterations=st.iterations, forceMax=st.forceMax,
+                 converged=bool(st.converged), energy_per_site=e / lat.n_active())
+        if mode == "mixed":
+            r.update(fp32_until=first_fc, fp32_iterations=first, fp32_seconds=fp32_seconds)
+        if ring and k == 0:
+            from qmin import defects
+            c = L / 2.0
+            r["defect_ring"] = defects.ring_locus(lat, (c, c, c), 20.0 * L / 256.0)
+        print("time to forceMax<%g (%s): %d iterations, %.2f s" % (fc, mode, st.iterations, elapsed),
+              file=sys.stderr, flush=True)
+        out.append(r)
     return out
+
+
+def time_to_tolerance(qmin, L, mode, fc=1e-8, switch_fc=None, max_iter=400000):
+    """One run of tolerance_chain to a single tolerance."""
+    return tolerance_chain(qmin, L, mode, (fc,), switch_fc, max_iter)[0]
 
 
 def warm_start(qmin, L, fc=1e-8):
@@ -146,12 +172,31 @@ def warm_start(qmin, L, fc=1e-8):
     return out
 
 
-def compare_modes(qmin, L, fc, switch_fc=None):
-    t64 = time_to_tolerance(qmin, L, "fp64", fc)
-    tmx = time_to_tolerance(qmin, L, "mixed", fc, switch_fc)
+def _pair(fc, t64, tmx):
     return {"forceMax_below": fc, "fp64": t64, "mixed_fp32_then_fp64_polish": tmx,
             "mixed_speedup": t64["seconds"] / tmx["seconds"],
             "energy_rel_diff": abs(tmx["energy_per_site"] - t64["energy_per_site"]) / abs(t64["energy_per_site"])}
+
+
+def compare_modes(qmin, L, fc=1e-8, tight=(1e-10, 1e-11), switch_fc=MIXED_SWITCH):
+    """FP64 against mixed precision to forceMax < fc (FP32 until fc, then the
+    FP64 polish) and, as one continued run per mode, to each tighter tolerance
+    (FP32 until switch_fc near its rounding floor).  Returns the fc comparison,
+    with the defect ring of the FP64 state, and the tight ones."""
+    fp64 = tolerance_chain(qmin, L, "fp64", (fc,) + tuple(tight), ring=True)
+    mixed = time_to_tolerance(qmin, L, "mixed", fc)
+    first = _pair(fc, fp64[0], mixed)
+    first["defect_ring"] = fp64[0].pop("defect_ring")
+    tight_out = {}
+    if tight:
+        mixed_tight = tolerance_chain(qmin, L, "mixed", tuple(tight), switch_fc)
+        for k, t in enumerate(tight):
+            tight_out["%g" % t] = _pair(t, fp64[k + 1], mixed_tight[k])
+        # how far the FP64 energy itself still moves between the tight tolerances
+        if len(tight) > 1:
+            e = [fp64[k + 1]["energy_per_site"] for k in range(len(tight))]
+            tight_out["fp64_energy_rel_change_%g_to_%g" % (tight[0], tight[-1])] = abs(e[-1] - e[0]) / abs(e[0])
+    return first, tight_out
 
 
 def ensemble_throughput(qmin, count=1024, fc=1e-7, max_iter=10000, sequential_sample=32, graph_iterations=32):
@@ -199,13 +244,17 @@ def ensemble_throughput(qmin, count=1024, fc=1e-7, max_iter=10000, sequential_sa
 # the ensemble's.  Online L-BFGS (no line search, energyMinimizerLoLBFGS.cpp:104-187)
 # converges at 128^3 with dt = 5e-4 but diverges at 384^3 for dt >= 1e-4 (after
 # about 780 iterations at 1e-4, 100 at 2e-4); at 5e-5 it stays finite and runs a
-# fixed number of iterations.  Adam keeps steps of size ~dt however small the
-# force (m/sqrt(s) does not shrink with F): at a constant dt it stalls near
-# forceMax ~ 1e-6 and also runs a fixed number of iterations.
+# fixed number of iterations.  With the reference's scheduledMomentum
+# (energyMinimizerLoLBFGS.cpp:205-208: step tau / (tau + iteration) x dt, tau = 10)
+# it runs at dt = 1e-4 with a step that shrinks as 1/iteration.  Adam keeps steps
+# of size ~dt however small the force (m/sqrt(s) does not shrink with F): at a
+# constant dt it stalls near forceMax ~ 1e-6 and also runs a fixed number of
+# iterations.
 FULL_ELASTICITY_MINIMIZERS = (
     ("FIRE", "FIRE", dict(dt=5e-4, dt_max=2e-2), 20000),
     ("Nesterov", "Nesterov", dict(dt=1e-2, mu=0.01), 20000),
     ("LoLBFGS(m=8)", "LoLBFGS", dict(m=8, dt=5e-5), 500),
+    ("LoLBFGS(m=8, scheduledMomentum tau=10)", "LoLBFGS", dict(m=8, dt=1e-4, tau=10.0, scheduled=True), 1000),
     ("Adam", "Adam", dict(dt=1e-3), 1000),
 )
 
@@ -270,10 +319,11 @@ def _full_elasticity_run(qmin, workloads, L, fc, cls, params, cap, mode):
     return r
 
 
-def measured_traffic(size, precision):
-    """HBM bytes per fused-pass dispatch from the committed rocprofv3 counter
-    summary of this workload, when it covers this size and precision."""
-    path = os.path.join(REPO, "profiles", "march_wide_kernel_fire_%s_%d.traffic.json" % (precision, size))
+def measured_traffic(kernel, size, precision):
+    """HBM bytes per dispatch of the FIRE lattice kernel from the committed
+    rocprofv3 counter summary of this workload, when it covers this kernel,
+    size and precision."""
+    path = os.path.join(REPO, "profiles", "%s_fire_%s_%d.traffic.json" % (kernel, precision, size))
     if not os.path.exists(path):
         return None
     with open(path) as fh:
@@ -350,7 +400,7 @@ def main():
                     help="also run every minimizer to tolerance on the 384^3 full-elasticity workload in FP64 and "
                          "mixed precision (several minutes)")
     ap.add_argument("--no-energy-check", action="store_true",
-                    help="skip the FP64-vs-mixed converged-energy comparison at forceMax < 1e-10")
+                    help="skip the FP64-vs-mixed comparisons at forceMax < 1e-10 and 1e-11")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build's hot path runs on a single MI355X "
@@ -367,6 +417,17 @@ def main():
     value = r["n"] * args.steps / r["wall"]
     bps = BYTES_PER_SITE[precision]
     achieved = bps * r["n"] / (r["march_ms"] * 1e-3) / 1e9
+    if args.size < 128:
+        kernel, kernel_desc, state_bytes = "march_kernel", "fused FIRE lattice pass (march_kernel, 32x8 / 64x4 " \
+                                                           "tile)", bps
+    elif precision == qmin.QM_FP64:
+        # the FIRE pass that keeps only Q and v: it reads and writes Q and v (and reads the site class), and
+        # recomputes the previous force on a one-site halo instead of storing F
+        kernel, state_bytes = "fire_stream_kernel", 161.0
+        kernel_desc = "FIRE pass keeping Q and v only (fire_stream_kernel, 32x16 tile)"
+    else:
+        kernel, kernel_desc, state_bytes = "march_wide_kernel", "fused FIRE lattice pass (march_wide_kernel, 64x8 " \
+                                                                "tile)", bps
     out = {
         "metric": METRIC,
         "value": value,
@@ -384,16 +445,20 @@ def main():
                                % (args.size, args.size - 1, 20.0 * args.size / 256.0),
                    "sites": r["n"], "boundary_sites": r["n_boundary"], "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": measured_traffic(args.size, args.precision),
-                     "kernel": "fused FIRE lattice pass (%s)" % (
-                         ("march_wide_kernel, %s tile" % ("32x16" if precision == qmin.QM_FP64 else "64x8"))
-                         if args.size >= 128 else "march_kernel, 32x8 / 64x4 tile"),
-                     "kernel_ms": r["march_ms"], "boundary_kernel_ms": r["boundary_ms"], "bytes_per_site": bps},
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": measured_traffic(kernel, args.size, args.precision),
+                     "kernel": kernel_desc, "kernel_ms": r["march_ms"], "boundary_kernel_ms": r["boundary_ms"],
+                     "bytes_per_site": bps, "kernel_state_bytes_per_site": state_bytes},
         "iteration_hbm_frac": value * bps / (PEAK_HBM_GBS * 1e9),
         "forceMax_after": r["forceMax"],
     }
     if not args.no_ttt:
-        out["time_to_tolerance"] = compare_modes(qmin, args.size, args.ttt_fc)
+        tight = () if args.no_energy_check else (1e-10, 1e-11)
+        out["time_to_tolerance"], checks = compare_modes(qmin, args.size, args.ttt_fc, tight)
+        if checks:
+            # forceMax < 1e-8 (= |F|_2 / N) leaves the slow elastic modes unconverged by ~1e-6 in E/N whatever
+            # the precision; the soft azimuthal mode of the director between the planar walls still moves the
+            # energy by a few 1e-9 between 1e-10 and 1e-11
+            out["converged_energy_check"] = checks
         ws = warm_start(qmin, args.size, args.ttt_fc)
         t_random = out["time_to_tolerance"]["fp64"]
         for v in ws.values():
@@ -401,10 +466,6 @@ def main():
             v["energy_rel_diff_vs_random_fp64"] = (v["energy_per_site"] - t_random["energy_per_site"]) / abs(
                 t_random["energy_per_site"])
         out["time_to_tolerance"]["warm_start"] = ws
-    if not args.no_energy_check:
-        # forceMax < 1e-8 (= |F|_2 / N) leaves the slow elastic modes unconverged by ~1e-6 in E/N whatever the
-        # precision; at 1e-10 the converged energies of the two modes are compared
-        out["converged_energy_check"] = compare_modes(qmin, args.size, 1e-10, switch_fc=1e-8)
     if args.ensemble:
         out["ensemble"] = ensemble_throughput(qmin)
     if args.full_elasticity:

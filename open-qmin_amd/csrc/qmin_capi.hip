@@ -881,12 +881,15 @@ static int upload_weights(qm_lattice* L, const qm_ldg* f) {
     return QM_OK;
 }
 
-// Tile shape of the fused pass: the one set on the lattice, else the measured
-// better default.  On lattices at least 128 sites wide in x and y: 32 x 16 in
-// FP64 (256^3: 2% faster than 32 x 8, 3% faster than 64 x 8) and 64 x 8 in FP32
-// (6% faster than 32 x 16, 11% faster than 64 x 4).  On smaller ones, and with
-// the L6 term (whose pass needs more than the 128 registers the 512-thread tiles
-// allow), 32 x 8 for FP64 and 64 x 4 for FP32.
+// Tile shape of the lattice pass: the one set on the lattice, else the measured
+// better default.  On lattices at least 128 sites wide in x and y: for FP64 FIRE
+// passes, the 32 x 16 pass that keeps only Q and v (256^3: 0.87-0.92 ms per
+// iteration against 0.93-0.96 for the fused 32 x 16 pass, which reads and
+// writes F as well); for FP64 force-only passes 32 x 16 (2% faster than 32 x 8,
+// 3% faster than 64 x 8); in FP32 64 x 8 (6% faster than 32 x 16, 11% faster
+// than 64 x 4, and 20% faster than the FP32 passes that keep only Q and v).
+// On smaller lattices, and with the L6 term (whose pass needs more than the 128
+// registers the 512-thread tiles allow), 32 x 8 for FP64 and 64 x 4 for FP32.
 // The tiles of the FIRE pass that keeps only Q and v (4..6) serve FIRE passes
 // without the L6 term; a force-only pass, or one with L6, then takes the
 // default of its kind.
@@ -894,8 +897,10 @@ int qmin::active_tile(const qm_lattice* L, int mode) {
     // the 64 x 8 FIRE tile that keeps only Q and v needs more LDS than a CU has in FP64
     const int t = (L->tileOverride == kStream64x8 && L->precision == QM_FP64) ? kStream32x16 : L->tileOverride;
     if (t >= 0 && !(is_stream_tile(t) && (mode != kFire || L->terms == kMultiL6))) return t;
-    if (L->terms != kMultiL6 && L->lx >= 128 && L->ly >= 128)
-        return L->precision == QM_FP64 ? kTile32x16 : kTile64x8;
+    if (L->terms != kMultiL6 && L->lx >= 128 && L->ly >= 128) {
+        if (L->precision == QM_FP64) return mode == kFire ? kStream32x16 : kTile32x16;
+        return kTile64x8;
+    }
     return L->precision == QM_FP64 ? kTile32x8 : kTile64x4;
 }
 

@@ -6,10 +6,12 @@
 // Kernels (BF16 operands on the matrix cores, FP32 accumulation):
 //  * conv_fwd_kernel: 3x3x3 dilated periodic convolution as an implicit GEMM,
 //    D[site][cout] = sum_{tap, cin} X[site + dil*tap][cin] Wt[tap][cout][cin],
-//    with v_mfma_f32_16x16x16_bf16: a wave owns 64 consecutive sites and all
-//    output channels; each lane feeds 4 consecutive input channels of one
-//    neighbour site (activations are channels-last, so that is one 8-byte load)
-//    and the tap's weights come from LDS.
+//    on the matrix cores: a wave owns 64 consecutive sites and all output
+//    channels; with 32 or 64 input channels each lane feeds 8 consecutive
+//    input channels of one neighbour site to v_mfma_f32_16x16x32_bf16
+//    (activations are channels-last, so that is one 16-byte load), with 16
+//    input channels 4 of them to v_mfma_f32_16x16x16_bf16; the tap's weights
+//    come from LDS.
 //    Optional ReLU on the input as it is read, bias, and an output mask (the
 //    ReLU derivative of the previous layer when the kernel runs backwards).
 //  * The input gradient is the same kernel with the taps mirrored and the
@@ -28,6 +30,14 @@ namespace ws {
 
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef unsigned v4u __attribute__((ext_vector_type(4)));
+
+// BF16 pairs with the sign bit set (negative values and -0) -> +0
+__device__ __forceinline__ unsigned relu2(unsigned w) {
+    const unsigned lo = (w & 0x8000u) ? 0xffffu : 0u, hi = (w & 0x80000000u) ? 0xffff0000u : 0u;
+    return w & ~(lo | hi);
+}
 
 constexpr int kTaps = 27;
 constexpr int kWsThreads = 256;
@@ -71,8 +81,11 @@ constexpr int kFwdMT = 4;   // 16-site row tiles per wave in the forward kernel
 // against bank conflicts), double-buffered across taps.
 template <int NT, int KT>
 __global__ void __launch_bounds__(kWsThreads) conv_fwd_kernel(ConvArgs a) {
-    constexpr int CO = NT * 16, CI = KT * 16, RS = CI + 4;   // LDS row stride in BF16
-    __shared__ uint2 sW[2][CO * RS / 4];
+    // K32: 32 input channels per matrix instruction; LDS rows padded by 8 BF16
+    // so that every lane's 16-byte weight fragment stays 16-byte aligned
+    constexpr bool K32 = KT % 2 == 0;
+    constexpr int CO = NT * 16, CI = KT * 16, RS = CI + (K32 ? 8 : 4);   // LDS row stride in BF16
+    __shared__ __align__(16) uint2 sW[2][CO * RS / 4];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int kq = lane >> 4, l16 = lane & 15;
     const long long wbase = ((long long)blockIdx.x * (kWsThreads / 64) + wave) * (16 * kFwdMT);
@@ -107,7 +120,35 @@ __global__ void __launch_bounds__(kWsThreads) conv_fwd_kernel(ConvArgs a) {
         const int buf = tap & 1;
         // the other buffer was last read in tap - 1, before the barrier below
         if (tap + 1 < kTaps) stage(tap + 1, buf ^ 1);
-        if (waveActive) {
+        if (K32 && waveActive) {
+            const uint16_t* xr[kFwdMT];
+#pragma unroll
+            for (int mt = 0; mt < kFwdMT; ++mt)
+                xr[mt] = a.x + tap_site(xs[mt], ys[mt], zs[mt], tap, a.dil, a.lx, a.ly, a.lz) * CI + kq * 8;
+#pragma unroll
+            for (int k2 = 0; k2 < KT / 2; ++k2) {
+                v8bf av[kFwdMT];
+#pragma unroll
+                for (int mt = 0; mt < kFwdMT; ++mt) {
+                    v4u raw = *reinterpret_cast<const v4u*>(xr[mt] + k2 * 32);
+                    if (a.reluIn) {
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) raw[j] = relu2(raw[j]);
+                    }
+                    if (!valid[mt]) raw = v4u{0u, 0u, 0u, 0u};
+                    av[mt] = __builtin_bit_cast(v8bf, raw);
+                }
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    const v4u wraw =
+                        *reinterpret_cast<const v4u*>(&sW[buf][((t * 16 + l16) * RS + k2 * 32 + kq * 8) / 4]);
+                    const v8bf bv = __builtin_bit_cast(v8bf, wraw);
+#pragma unroll
+                    for (int mt = 0; mt < kFwdMT; ++mt)
+                        acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[mt], bv, acc[mt][t], 0, 0, 0);
+                }
+            }
+        } else if (waveActive) {
             const uint16_t* xr[kFwdMT];
 #pragma unroll
             for (int mt = 0; mt < kFwdMT; ++mt)
@@ -180,15 +221,16 @@ constexpr int kWgLd = kWgSites + 8;      // LDS row stride (BF16) of the transpo
 
 // One work-group per (chunk of sites, tap).  Per step it stages 64 sites of the
 // output gradient G and of the tap's neighbours of X (ReLU applied) in LDS,
-// transposed to [channel][site], so that both MFMA operands read 4 consecutive
-// sites (the contraction axis) as one 8-byte LDS load; the four waves split the
-// 16 x 16 output tiles of dW[tap][cout][cin].
+// transposed to [channel][site], so that both operands of
+// v_mfma_f32_16x16x32_bf16 read 8 consecutive sites (the contraction axis) as
+// one 16-byte LDS load; the four waves split the 16 x 16 output tiles of
+// dW[tap][cout][cin].
 template <int NT, int KT>
 __global__ void __launch_bounds__(kWsThreads) conv_wgrad_kernel(WgradArgs a) {
     constexpr int CO = NT * 16, CI = KT * 16, TILES = NT * KT;
     constexpr int TPW = (TILES + 3) / 4;    // tiles per wave
-    __shared__ uint16_t gT[CO * kWgLd];
-    __shared__ uint16_t xT[CI * kWgLd];
+    __shared__ __align__(16) uint16_t gT[CO * kWgLd];
+    __shared__ __align__(16) uint16_t xT[CI * kWgLd];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int kq = lane >> 4, l16 = lane & 15;
     const int tap = blockIdx.y;
@@ -233,20 +275,12 @@ __global__ void __launch_bounds__(kWsThreads) conv_wgrad_kernel(WgradArgs a) {
             if (tile >= TILES) break;
             const int mt = tile / KT, nt = tile % KT;
 #pragma unroll
-            for (int ks = 0; ks < kWgSites / 16; ++ks) {
-                const int site = ks * 16 + 4 * kq;
-                const uint2 ar = *reinterpret_cast<const uint2*>(&gT[(mt * 16 + l16) * kWgLd + site]);
-                const uint2 br = *reinterpret_cast<const uint2*>(&xT[(nt * 16 + l16) * kWgLd + site]);
-                v4s av, bv;
-                av[0] = short(ar.x & 0xffffu);
-                av[1] = short(ar.x >> 16);
-                av[2] = short(ar.y & 0xffffu);
-                av[3] = short(ar.y >> 16);
-                bv[0] = short(br.x & 0xffffu);
-                bv[1] = short(br.x >> 16);
-                bv[2] = short(br.y & 0xffffu);
-                bv[3] = short(br.y >> 16);
-                acc[k] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[k], 0, 0, 0);
+            for (int ks = 0; ks < kWgSites / 32; ++ks) {
+                const int site = ks * 32 + 8 * kq;
+                const v4u ar = *reinterpret_cast<const v4u*>(&gT[(mt * 16 + l16) * kWgLd + site]);
+                const v4u br = *reinterpret_cast<const v4u*>(&xT[(nt * 16 + l16) * kWgLd + site]);
+                acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, ar),
+                                                                 __builtin_bit_cast(v8bf, br), acc[k], 0, 0, 0);
             }
         }
         __syncthreads();

@@ -1,7 +1,8 @@
 """Fixed workloads for hardware-counter and kernel-trace profiling.
 
-  python open-qmin_amd/qmin/profile_run.py fire [--size 256] [--precision fp64] [--steps 10]
-      FIRE iterations on the 256^3 walls + sphere workload (as bench.py).
+  python open-qmin_amd/qmin/profile_run.py fire [--size 256] [--precision fp64] [--steps 10] [--tile T]
+      FIRE iterations on the 256^3 walls + sphere workload (as bench.py), with
+      the library's default tile or tile T (qm_lattice_set_tile).
   python open-qmin_amd/qmin/profile_run.py copy [--mib 512] [--steps 10]
       a plain device-to-device copy of known size, used to calibrate the
       FETCH_SIZE / WRITE_SIZE counters (bytes the counter reports per byte moved).
@@ -110,6 +111,7 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--mib", type=int, default=512)
+    ap.add_argument("--tile", type=int, default=-1, help="tile of the FIRE lattice pass (-1: the library's default)")
     args = ap.parse_args()
     if args.what == "copy":
         import torch
@@ -141,6 +143,8 @@ def main():
     from bench import build_workload, fire_params
     precision = qmin.QM_FP64 if args.precision == "fp64" else qmin.QM_FP32
     lat, force = build_workload(qmin, args.size, precision)
+    if args.tile >= 0:
+        lat.set_tile(args.tile)
     m = qmin.FIRE(fc=1e-8, max_iter=10 ** 9, **fire_params())
     m.run_iterations(force, lat, args.steps)
     print("ran %d FIRE iterations on %d^3 (%s), forceMax %.4g" % (args.steps, args.size, args.precision,
