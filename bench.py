@@ -135,16 +135,18 @@ def time_to_tolerance(qmin, L, mode, fc=1e-8, switch_fc=None, max_iter=400000):
     return tolerance_chain(qmin, L, mode, (fc,), switch_fc, max_iter)[0]
 
 
-def warm_start(qmin, L, fc=1e-8):
+WEIGHTS = os.path.join(REPO, "open-qmin_amd", "warmstart", "weights_64.npz")
+
+
+def warm_start(qmin, L, fc=1e-8, weights=WEIGHTS):
     """FIRE to forceMax < fc on the bench workload from the trained network's
     prediction (committed weights) and from a uniform director along the same
     far-field direction (x, in the plane of the planar walls), in FP64; and from
     the prediction in mixed precision (FP32 to fc, then the FP64 polish as in
     time_to_tolerance).  Times include the prediction."""
     from qmin import warmstart as ws
-    path = os.path.join(REPO, "open-qmin_amd", "warmstart", "weights_64.npz")
     net = ws.WarmStartNet(seed=0)
-    net.load(path)
+    net.load(weights)
     A, B, C, S0 = phase_constants()
     out = {}
     for mode in ("network", "uniform_director", "network_mixed"):
@@ -169,6 +171,52 @@ def warm_start(qmin, L, fc=1e-8):
             out[mode].update(fp32_iterations=first, polish_iterations=st.iterations - first)
         print("time to forceMax<%g (%s start): %d iterations, %.2f s" % (fc, mode, st.iterations, t1 - t0),
               file=sys.stderr, flush=True)
+    return out
+
+
+def warm_start_heldout(qmin, L=128, seeds=(12, 13), random_seed=4321, fc=1e-8, weights=WEIGHTS):
+    """The learned warm start on held-out periodic L^3 geometries with 2-6
+    spheres (qmin.warmstart.heldout_geometry; 5 and 6 spheres for seeds 12 and
+    13): FIRE to forceMax < fc from a seeded random start, from a uniform
+    director along a body diagonal, and from the network's prediction along the
+    same diagonal (time includes the prediction), all with FP32 state and FP64
+    reductions."""
+    from qmin import warmstart as ws
+    net = ws.WarmStartNet(seed=0)
+    net.load(weights)
+    A, B, C, S0 = phase_constants()
+    force = qmin.LandauDeGennesLC(A, B, C, 2.32, 0.5, 0.3, semantics=qmin.SEMANTICS_REF_GPU)
+    d = ws.diagonal_director()
+    out = {}
+    for seed in seeds:
+        spheres = ws.heldout_geometry(L, seed)
+        res = {"spheres": len(spheres)}
+        for mode in ("random", "uniform_diagonal", "network_diagonal"):
+            lat = qmin.Lattice(L, L, L, precision=qmin.QM_FP32)
+            if mode == "random":
+                lat.random_init(random_seed, S0)
+            else:
+                lat.uniform_init(d, S0)
+            for c, r, t, W in spheres:
+                lat.sphere(c, r, t, W, S0)
+            t0 = time.perf_counter()
+            if mode.startswith("network"):
+                net.predict(lat, d, S0)
+            st = qmin.FIRE(dt=5e-4, dt_max=2e-2, fc=fc, max_iter=400000).minimize(force, lat)
+            t1 = time.perf_counter()
+            e, _ = force.compute_energy(lat)
+            res[mode] = dict(seconds=t1 - t0, iterations=st.iterations, forceMax=st.forceMax,
+                             converged=bool(st.converged), energy_per_site=e / lat.n_active())
+            print("held-out %d^3 seed %d, %s start: %d iterations, %.2f s" % (L, seed, mode, st.iterations, t1 - t0),
+                  file=sys.stderr, flush=True)
+            del lat
+        er = res["random"]["energy_per_site"]
+        for mode in ("uniform_diagonal", "network_diagonal"):
+            res[mode]["speedup_vs_random"] = res["random"]["seconds"] / res[mode]["seconds"]
+            res[mode]["energy_rel_diff_vs_random"] = (res[mode]["energy_per_site"] - er) / abs(er)
+        res["network_diagonal"]["speedup_vs_uniform"] = res["uniform_diagonal"]["seconds"] / \
+            res["network_diagonal"]["seconds"]
+        out["seed_%d" % seed] = res
     return out
 
 
@@ -466,6 +514,7 @@ def main():
             v["energy_rel_diff_vs_random_fp64"] = (v["energy_per_site"] - t_random["energy_per_site"]) / abs(
                 t_random["energy_per_site"])
         out["time_to_tolerance"]["warm_start"] = ws
+        out["time_to_tolerance"]["warm_start_heldout_128"] = warm_start_heldout(qmin)
     if args.ensemble:
         out["ensemble"] = ensemble_throughput(qmin)
     if args.full_elasticity:

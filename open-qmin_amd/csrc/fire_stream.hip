@@ -29,6 +29,12 @@ namespace qmin {
 
 namespace {
 
+// Points of the composed operator between scheduling barriers.  At 256^3 FP64
+// a barrier every 2 points gave 0.802 ms per pass against 0.823-0.825 ms for
+// 4, 6 or 8 (and 168 registers with spills for none): more LDS reads in flight
+// do not pay, as the pass is bound by latency between its barriers.
+constexpr int kSchedPoints = 2;
+
 __device__ __forceinline__ int swrap(int v, int m) {
     v %= m;
     return v < 0 ? v + m : v;
@@ -92,9 +98,9 @@ __device__ __forceinline__ void direct_spatial(SConstPtr<R> W, const R (*q)[5][N
             s[k] = R(6) * qc[k] - q[sc][k][h - 1] - q[sc][k][h + 1] - q[sc][k][h - HXS] - q[sc][k][h + HXS] - cm[k] -
                    cp[k];
     } else if constexpr (TERMS == kMultiLinear) {
-        ldg_ring_lin_a<HXS, NP>(W, q, sm, sc, sp, h, cm, qc, cp, s);
+        ldg_ring_lin_a<HXS, NP, kSchedPoints>(W, q, sm, sc, sp, h, cm, qc, cp, s);
     } else {
-        ldg_ring_lin_b<HXS, NP>(W, q, sm, sc, sp, h, cm, qc, cp, s);
+        ldg_ring_lin_b<HXS, NP, kSchedPoints>(W, q, sm, sc, sp, h, cm, qc, cp, s);
     }
 }
 

@@ -22,7 +22,7 @@ import time
 
 import numpy as np
 
-from . import QM_FP32, Ensemble, FIRE, _check, library
+from . import QM_FP32, Ensemble, FIRE, Lattice, _check, library
 from . import workloads
 
 LAYERS = ((16, 32, 1), (32, 64, 2), (64, 64, 4), (64, 16, 8))   # (cin, cout, dilation)
@@ -76,14 +76,31 @@ def far_field_director(q, types):
 
 class Sample:
     """Device-resident training pair: features, converged Q, site types, and the
-    uniform far-field Q the network's output corrects."""
+    uniform far-field Q the network's output corrects; the geometry's spec, from
+    which the energy term of the loss builds a lattice when none is kept."""
 
-    def __init__(self, dims, feats, target, types, offset, lattice=None):
+    def __init__(self, dims, feats, target, types, offset, lattice=None, spec=None):
         self.dims, self.feats, self.target, self.types = dims, feats, target, types
         self.offset = np.ascontiguousarray(offset, np.float64)
         self.n = dims[0] * dims[1] * dims[2]
         self.n_lc = int((types.cpu().numpy() <= 0).sum()) if hasattr(types, "cpu") else 0
         self.lattice = lattice   # the lattice with this geometry (for the energy term), or None
+        self.spec = spec
+
+    def energy_lattice(self):
+        """A lattice with this sample's geometry (object sites hold their
+        boundary Q; the liquid-crystal sites are overwritten by the prediction)."""
+        if self.lattice is not None:
+            return self.lattice
+        if self.spec is None:
+            return None
+        S0 = workloads.phase_constants()[3]
+        lat = Lattice(*self.spec["dims"], precision=QM_FP32)
+        for xyz, plane, wt, W in self.spec.get("walls", []):
+            lat.wall(xyz, plane, wt, W, S0)
+        for centre, r, btype, W in self.spec["spheres"]:
+            lat.sphere(centre, r, btype, W, S0)
+        return lat
 
 
 def lattice_features(lat, director):
@@ -209,19 +226,28 @@ class WarmStartNet:
                                        _dp(sample.offset), _ptr(g), _ptr(part), blocks, _stream()))
         return g, part
 
-    def energy_grad(self, y, sample, force, weight):
-        """Adds the energy term of the loss, weight * E(prediction) / (5 n_lc):
-        the prediction goes into the sample's lattice, whose force F (minus the
-        energy gradient, shaped by the metric) gives -weight F / (5 n_lc)."""
-        _check(self.lib.qm_ws_apply(sample.lattice.h, _ptr(y), 16, _dp(sample.offset)))
-        return weight / max(sample.n_lc * 5, 1)
+    def energy_grad(self, y, g, sample, force, weight):
+        """Adds the gradient of the energy term of the loss, weight x E(prediction)
+        / (5 n_lc), to g: the prediction goes into a lattice of the sample's
+        geometry, whose force F (minus the energy gradient, shaped by the metric,
+        from the library's force pass) adds -weight F / (5 n_lc).  Returns the
+        term's value (energy per liquid-crystal component, times weight)."""
+        lat = sample.energy_lattice()
+        if lat is None:
+            return 0.0
+        _torch().cuda.synchronize()
+        _check(self.lib.qm_ws_apply(lat.h, _ptr(y), 16, _dp(sample.offset)))
+        scale = weight / max(sample.n_lc * 5, 1)
+        _check(self.lib.qm_ws_energy_grad(force.h, lat.h, scale, _ptr(g), 16, _stream()))
+        e, _ = force.compute_energy(lat)
+        return weight * e / max(sample.n_lc * 5, 1)
 
     def train_step(self, sample, lr=1e-3, force=None, energy_weight=0.0):
         acts = self.forward(sample.feats, sample.dims)
         g, part = self.loss_and_grad(acts[-1], sample)
-        if energy_weight > 0 and force is not None and sample.lattice is not None:
-            scale = self.energy_grad(acts[-1], sample, force, energy_weight)
-            _check(self.lib.qm_ws_energy_grad(force.h, sample.lattice.h, scale, _ptr(g), 16, _stream()))
+        self.last_energy_term = 0.0
+        if energy_weight > 0 and force is not None:
+            self.last_energy_term = self.energy_grad(acts[-1], g, sample, force, energy_weight)
         self.backward(acts, g, sample.dims)
         self.adam_step(lr)
         return part
@@ -236,12 +262,15 @@ class WarmStartNet:
         history = []
         for ep in range(epochs):
             order = rng.permutation(len(samples))
-            parts = [self.train_step(samples[k], lr, force, energy_weight) for k in order]
+            parts, eterm = [], 0.0
+            for k in order:
+                parts.append(self.train_step(samples[k], lr, force, energy_weight))
+                eterm += self.last_energy_term
             torch.cuda.synchronize()
             mse = sum(float(p.sum().item()) / max(samples[k].n_lc * 5, 1) for p, k in zip(parts, order)) / len(order)
             history.append(mse)
             if log:
-                log("epoch %d: mean squared error %.4e" % (ep + 1, mse))
+                log("epoch %d: mean squared error %.4e, energy term %.6e" % (ep + 1, mse, eterm / len(order)))
         return history
 
     def predict(self, lat, director, S0=None):
@@ -254,6 +283,40 @@ class WarmStartNet:
         torch.cuda.synchronize()
         _check(self.lib.qm_ws_apply(lat.h, _ptr(acts[-1]), 16, _dp(uniform_q(director, S0))))
         return acts[-1]
+
+
+# ---------------------------------------------------------------- geometries
+def diagonal_director():
+    """Far-field director of predictions on periodic lattices: a body diagonal
+    of the cubic lattice.  The discrete operators are not isotropic, and around
+    objects the texture's energy depends on how the far field sits against the
+    lattice axes; minimized states on periodic lattices end with far fields near
+    the body diagonals, and the far field turns towards them only slowly (a soft
+    mode: a start along z on a 128^3 lattice with five spheres took 104k FIRE
+    iterations, one along the far field it ended with 15.6k).  Predicting along a
+    diagonal skips most of that turn."""
+    return tuple(float(v) for v in np.ones(3) / math.sqrt(3.0))
+
+
+def heldout_geometry(L, seed):
+    """Spheres of a held-out periodic L^3 geometry: 2-6 spheres at seeded
+    positions, radius uniform in [L/16, L/8], homeotropic or degenerate planar,
+    W log-uniform in [0.5, 20]: [(centre, radius, btype, W)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(int(rng.integers(2, 7))):
+        c = tuple(float(v) for v in rng.uniform(0, L, 3))
+        out.append((c, float(rng.uniform(L / 16, L / 8)), int(rng.integers(0, 2)),
+                    float(math.exp(rng.uniform(math.log(0.5), math.log(20.0))))))
+    return out
+
+
+def random_start_specs(count, size=64, seed=4096):
+    """Training geometries as the ensemble workload's at size^3 (periodic, 0-3
+    spheres of radius 4-16, homeotropic or degenerate planar, W log-uniform in
+    [0.5, 20]) with seeded random-director starts: the targets are the states
+    the ensemble minimizer reaches from them."""
+    return workloads.ensemble_specs(count, seed, edges=(size,))
 
 
 # ---------------------------------------------------------------- training data
@@ -308,12 +371,12 @@ def make_dataset(qmin, specs, force, fc=1e-7, max_iter=4000, batch=128, log=None
         lats = [build_lattice(qmin, s, QM_FP32, S0) for s in part]
         t0 = time.perf_counter()
         its, fms = Ensemble(lats).fire_minimize(FIRE(dt=5e-4, dt_max=2e-2, fc=fc, max_iter=max_iter), force, 32)
-        for lat in lats:
+        for lat, spec in zip(lats, part):
             q = lat.q()
             types = lat.types()
             d = far_field_director(q, types)
             s = Sample(lat.shape, lattice_features(lat, d), lattice_q(lat), lattice_types(lat), uniform_q(d, S0),
-                       lat if keep_lattices else None)
+                       lat if keep_lattices else None, spec)
             samples.append(s)
         if log:
             log("minimized %d training lattices in %.1f s (iterations mean %.0f, converged %d)"

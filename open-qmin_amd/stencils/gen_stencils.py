@@ -215,8 +215,10 @@ def gen_direct_distinct(name, keys, rows, distinct):
     (q[slot][c][NP], row stride HXS), with the distinct weights (a table per
     variant, emitted for the host); the site's own column (the points (0, 0, -1), (0, 0, 0)
     and (0, 0, 1)) comes from registers cm, cc, cp instead of the ring:
-    ldg_ring_lin_<name>(W, q, sm, sc, sp, h, cm, cc, cp, acc)."""
-    lines = ["template <int HXS, int NP, typename R, typename WP>",
+    ldg_ring_lin_<name><HXS, NP, SB>(W, q, sm, sc, sp, h, cm, cc, cp, acc): a
+    scheduling barrier after every SB points (0: none) bounds how many of the
+    ring's values the compiler keeps in flight."""
+    lines = ["template <int HXS, int NP, int SB, typename R, typename WP>",
              "__device__ __forceinline__ void ldg_ring_lin_%s(WP W, const R (*q)[5][NP], int sm, int sc, int sp,"
              " int h, const R* cm, const R* cc, const R* cp, R* acc) {" % name,
              "    R v;"]
@@ -225,8 +227,9 @@ def gen_direct_distinct(name, keys, rows, distinct):
     points_seen = []
     for (r, p, c), row in zip(keys, rows):
         if p not in points_seen:
-            if points_seen and len(points_seen) % 2 == 0:
-                lines.append("    __builtin_amdgcn_sched_barrier(0);")
+            if points_seen:
+                lines.append("    if constexpr (SB > 0 && %d %% SB == 0) __builtin_amdgcn_sched_barrier(0);"
+                             % len(points_seen))
             points_seen.append(p)
         if (p, c) != last:
             if p in own:

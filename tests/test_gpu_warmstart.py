@@ -167,6 +167,61 @@ def test_features_and_apply_round_trip():
     assert np.abs(lat.v()).max() == 0
 
 
+def test_energy_term_gradient_is_the_force():
+    """qm_ws_energy_grad, the gradient of the loss's energy term: with the
+    prediction written into the lattice, the first five of 16 BF16 channels of g
+    get -scale F at liquid-crystal sites (F from the oracle at the same Q, so
+    one BF16 rounding), object sites and the other channels keep their values,
+    and a small step of the prediction along -g lowers the energy."""
+    import pyoracle as po
+    from qmin import warmstart as ws
+    A, B, C = po.rescaled_phase()
+    S0 = po.uniaxial_s0(A, B, C)
+    shape = (20, 18, 16)
+    lat = qmin.Lattice(*shape)
+    lat.random_init(8, S0)
+    lat.sphere((10.0, 9.0, 8.0), 4.0, qmin.HOMEOTROPIC, 5.3, S0)
+    f = qmin.LandauDeGennesLC(A, B, C, 2.32, 0.5, 0.3, semantics=qmin.SEMANTICS_REF_GPU)
+    n = lat.n
+    rng = np.random.default_rng(3)
+    yq = (0.05 * rng.standard_normal((n, 16))).astype(np.float32)
+    y = torch.from_numpy(yq.reshape(-1).copy()).cuda()
+    off = ws.uniform_q((1.0, 1.0, 1.0), S0)
+    offp = off.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    lib = qmin.library()
+    torch.cuda.synchronize()
+    check(lib.qm_ws_apply(lat.h, ptr(y), 16, offp))
+    q = lat.q()
+    o = po.OracleSystem(*shape)
+    o.random_init(8, S0)
+    o.sphere((10.0, 9.0, 8.0), 4.0, 0, 5.3, S0)
+    o.set_q(q)
+    o.set_params(A, B, C, 2.32, 0.5, 0.3, multi=True, semantics=1)
+    fref = o.forces()
+    g0 = (0.01 * rng.standard_normal((n, 16))).astype(np.float32)
+    g = dev_bf16(g0.reshape(-1))
+    scale = 0.37
+    check(lib.qm_ws_energy_grad(f.h, lat.h, ctypes.c_float(scale), ptr(g), 16, stream()))
+    got = host_bf16(g).reshape(n, 16)
+    base = co.bf16_round(g0.reshape(-1)).reshape(n, 16)
+    lc = lat.types() <= 0
+    want = co.bf16_round((base[:, :5] - np.float32(scale) * fref.astype(np.float32)).astype(np.float32).reshape(-1))
+    want = want.reshape(n, 5)
+    ulp = np.abs(want) * 2.0 ** -7 + 1e-30
+    assert np.all(np.abs(got[lc, :5] - want[lc]) <= ulp[lc])
+    assert np.array_equal(got[~lc], base[~lc])
+    assert np.array_equal(got[:, 5:], base[:, 5:])
+    # descent: the prediction moved along +F (-g without the base) lowers the energy
+    e0, _ = f.compute_energy(lat)
+    y2 = yq.copy()
+    y2[:, :5] += (1e-3 * fref).astype(np.float32)
+    y.copy_(torch.from_numpy(y2.reshape(-1)))
+    torch.cuda.synchronize()
+    check(lib.qm_ws_apply(lat.h, ptr(y), 16, offp))
+    e1, _ = f.compute_energy(lat)
+    assert e1 < e0
+
+
 def test_training_reduces_the_loss():
     from qmin import warmstart as ws
     A, B, C, S0 = qmin.workloads.phase_constants()
