@@ -1,8 +1,10 @@
 """Fixed workloads for hardware-counter and kernel-trace profiling.
 
   python open-qmin_amd/qmin/profile_run.py fire [--size 256] [--precision fp64] [--steps 10] [--tile T]
-      FIRE iterations on the 256^3 walls + sphere workload (as bench.py), with
-      the library's default tile or tile T (qm_lattice_set_tile).
+         [--workload bench|full]
+      FIRE iterations on the 256^3 walls + sphere workload (as bench.py), or on
+      the full-elasticity workload, with the library's default tile or tile T
+      (qm_lattice_set_tile).
   python open-qmin_amd/qmin/profile_run.py copy [--mib 512] [--steps 10]
       a plain device-to-device copy of known size, used to calibrate the
       FETCH_SIZE / WRITE_SIZE counters (bytes the counter reports per byte moved).
@@ -112,6 +114,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--mib", type=int, default=512)
     ap.add_argument("--tile", type=int, default=-1, help="tile of the FIRE lattice pass (-1: the library's default)")
+    ap.add_argument("--workload", choices=["bench", "full"], default="bench",
+                    help="fire: the bench workload, or the full-elasticity one (L1-L4, L6, E field, four spheres)")
     args = ap.parse_args()
     if args.what == "copy":
         import torch
@@ -142,7 +146,11 @@ def main():
         return
     from bench import build_workload, fire_params
     precision = qmin.QM_FP64 if args.precision == "fp64" else qmin.QM_FP32
-    lat, force = build_workload(qmin, args.size, precision)
+    if args.workload == "full":
+        from qmin import workloads
+        lat, force = workloads.full_elasticity(qmin, args.size, precision)
+    else:
+        lat, force = build_workload(qmin, args.size, precision)
     if args.tile >= 0:
         lat.set_tile(args.tile)
     m = qmin.FIRE(fc=1e-8, max_iter=10 ** 9, **fire_params())

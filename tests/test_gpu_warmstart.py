@@ -258,3 +258,54 @@ def test_trained_network_beats_random_start():
         assert st.converged
         its[mode] = st.iterations
     assert its["network"] * 3 < its["random"], its
+
+
+def _heldout_iterations(seed, mode, L=128):
+    """FIRE iterations to forceMax < 1e-8 on the held-out periodic geometry
+    warmstart.heldout_geometry(L, seed) (FP32 state, FP64 reductions) from the
+    seeded random start, a uniform director along a body diagonal, or the
+    committed network's prediction along that diagonal; and the final E/N."""
+    import os
+    from qmin import warmstart as ws
+    A, B, C, S0 = qmin.workloads.phase_constants()
+    force = qmin.LandauDeGennesLC(A, B, C, 2.32, 0.5, 0.3, semantics=qmin.SEMANTICS_REF_GPU)
+    d = ws.diagonal_director()
+    lat = qmin.Lattice(L, L, L, precision=qmin.QM_FP32)
+    if mode == "random":
+        lat.random_init(4321, S0)
+    else:
+        lat.uniform_init(d, S0)
+    for c, r, t, W in ws.heldout_geometry(L, seed):
+        lat.sphere(c, r, t, W, S0)
+    if mode == "network":
+        net = ws.WarmStartNet()
+        net.load(os.path.join(os.path.dirname(qmin.__file__), "..", "warmstart", "weights_64.npz"))
+        net.predict(lat, d, S0)
+    st = qmin.FIRE(dt=5e-4, dt_max=2e-2, fc=1e-8, max_iter=200000).minimize(force, lat)
+    assert st.converged
+    e, _ = force.compute_energy(lat)
+    return st.iterations, e / lat.n_active()
+
+
+def test_network_start_beats_random_start_on_heldout_spheres():
+    """Held-out 128^3 periodic lattice with five spheres of either anchoring
+    (seed 12): FIRE from the committed network's prediction needs under a third
+    of the iterations of the seeded random start (measured 4,198 against
+    56,613)."""
+    print("held-out 128^3, 5 spheres: network against random start", flush=True)
+    net_its, _ = _heldout_iterations(12, "network")
+    rnd_its, _ = _heldout_iterations(12, "random")
+    assert 3 * net_its < rnd_its, (net_its, rnd_its)
+
+
+def test_network_start_beats_uniform_start_on_heldout_spheres():
+    """Held-out 128^3 periodic lattice with six spheres (seed 13): the network's
+    corrections near the objects save FIRE iterations over a uniform director
+    along the same diagonal (measured 19,513 against 32,112); the two smooth
+    starts end in nearby minima (E/N 1.6e-6 relative apart)."""
+    print("held-out 128^3, 6 spheres: network against uniform start", flush=True)
+    net_its, e_net = _heldout_iterations(13, "network")
+    uni_its, e_uni = _heldout_iterations(13, "uniform")
+    assert net_its < 0.8 * uni_its, (net_its, uni_its)
+    assert abs(e_net - e_uni) <= 1e-5 * abs(e_uni)
+
