@@ -17,9 +17,11 @@
 // the reference's expressions read from its source, and closed forms and
 // invariances (plane wave, lattice rotations, locality, first-derivative rules
 // on a quadratic field) for the rest.
-// Parity unpinned: orc_object_force's per-site area vector and sum (checked by
-// symmetry and translation properties only; the reference's force class that
-// computes them needs MPI and is not compiled here).
+// Parity unpinned: orc_object_force's area vector at sites with objects on
+// both sides of one axis (checked with the force on spheres by symmetry and
+// translation properties only; the reference's force class that computes it
+// needs MPI and is not compiled here).  Flat walls and corners are pinned in
+// closed form.
 #pragma once
 #include <stdint.h>
 

@@ -401,3 +401,57 @@ def test_object_force_on_a_flat_wall_closed_form(wall_type):
     sz_high = density(lz - 2, lz - 1, 1 - wall_type, 2.7) - L1 * trG2
     np.testing.assert_allclose(o.object_force(0), [0.0, 0.0, -lx * ly * sz_low], rtol=1e-12, atol=1e-14)
     np.testing.assert_allclose(o.object_force(1), [0.0, 0.0, lx * ly * sz_high], rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize("types_xz", [(0, 1), (1, 0)])
+def test_object_force_at_a_corner_closed_form(types_xz):
+    """computeObjectForces' area vector (landauDeGennesLCOtherForces.cpp:160-176)
+    at sites touching objects along two axes, in closed form.  Object 1 is a
+    wall at z = 0, object 2 a wall at x = 0, object 3 a wall at z = Lz - 1 (so
+    that no surface site of object 1 wraps round in z); a later wall takes the
+    sites it shares with an earlier one.  The liquid crystal holds
+    Q(x) = Q0 + (x - 1) G, uniform in y and z, so at object 1's surface sites
+    (z = 1) only the x derivative is nonzero, G (central, or one-sided next to
+    object 2 on either side), and the stress is diagonal: sigma_xx = e - L1 Tr G^2,
+    sigma_yy = sigma_zz = e, e the site's energy density.  Every site contributes
+    a_z = -1; the sites at x = 1 and at x = Lx - 1 (next to object 2 across the
+    periodic boundary) also a_x = -1 and +1, although object 2 is another
+    object (any object sets the area vector).  So object 1's force is
+    (Ly (e(Lx-1) - e(1)), 0, -Ly sum_x e(x))."""
+    lib = po.ref_lib()
+    lx, ly, lz = 7, 4, 6
+    L1 = 2.32
+    bt1, bt2 = types_xz
+    o = po.OracleSystem(lx, ly, lz)
+    o.random_init(9, S0)
+    o.wall(2, 0, bt1, 5.3, S0)
+    o.wall(0, 0, bt2, 2.7, S0)
+    o.wall(2, lz - 1, 1, 1.9, S0)
+    o.set_params(A, B, C, L1, semantics=1)
+    t = o.types().reshape(lz, ly, lx)
+    assert np.all(t[0, :, 1:] == 1) and np.all(t[:lz - 1, :, 0] == 2) and np.all(t[lz - 1] == 3)
+    q0 = np.zeros(5)
+    lib.qmref_q_from_director(po.dptr(np.array([0.0, 0.6, 0.8])), S0, po.dptr(q0))
+    G = np.array([0.011, 0.0, 0.0, -0.017, 0.0])
+    Q = o.q().reshape(lz, ly, lx, 5)
+    qb1 = np.ascontiguousarray(Q[0, 0, 3])
+    qb2 = np.ascontiguousarray(Q[2, 0, 0])
+    for x in range(1, lx):
+        Q[1:lz - 1, :, x] = q0 + (x - 1) * G
+    o.set_q(Q.reshape(-1, 5))
+    trG2 = 2 * (G[0] ** 2 + G[3] ** 2 + G[0] * G[3])
+
+    def density(x):
+        q = np.ascontiguousarray(Q[1, 0, x])
+        d, tr = np.zeros(15), np.zeros(3)
+        lib.qmref_phase(po.dptr(q), po.dptr(d), po.dptr(tr))
+        e = 0.5 * A * tr[0] + B / 3.0 * tr[1] + 0.25 * C * tr[2] + 0.5 * L1 * trG2
+        e += lib.qmref_anchoring_energy(bt1, 5.3, S0, po.dptr(q), po.dptr(qb1))
+        if x in (1, lx - 1):
+            e += lib.qmref_anchoring_energy(bt2, 2.7, S0, po.dptr(q), po.dptr(qb2))
+        return e
+
+    e = {x: density(x) for x in range(1, lx)}
+    want = [ly * (e[lx - 1] - e[1]), 0.0, -ly * sum(e.values())]
+    assert abs(want[0]) > 1e-3 * abs(want[2])
+    np.testing.assert_allclose(o.object_force(0), want, rtol=1e-12, atol=1e-13)
