@@ -9,6 +9,8 @@
 // on the tile from Q_{k+1}, and writes Q_{k+1} and v_{k+1} (160 B per FP64 site).
 // Sites that the boundary pass finishes (next to objects, class < 0) keep their
 // F in memory: the boundary pass writes it and this pass reads it on the halo.
+// With storeF (the last pass of a run of known length) F_{k+1} of the other
+// sites is stored too, so that the lattice holds the final state's force.
 //
 // Streaming: a work-group owns a TX x TY column and marches along z.  Each step
 // stages one plane of Q_k over the tile plus two sites on each side into a ring
@@ -269,6 +271,10 @@ __device__ __forceinline__ void stream_body(const MarchArgs<R>& A, const int log
                     for (int k = 0; k < 5; ++k) {
                         __builtin_nontemporal_store(q1[k], &out.q[k * n + i]);
                         __builtin_nontemporal_store(u[k], &out.u[k * n + i]);
+                    }
+                    if (A.storeF) {
+#pragma unroll
+                        for (int k = 0; k < 5; ++k) __builtin_nontemporal_store(F[k], &out.f[k * n + i]);
                     }
                     pF += qt_dot(F, F);
                     pP += qt_dot(F, u);

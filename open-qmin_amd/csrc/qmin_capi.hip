@@ -993,8 +993,10 @@ static int march_blocks(const qm_lattice* L, int mode) {
 // One pass (force only or one FIRE iteration); kernels only, no host sync.
 // partials: the work-group partial sums (null: the lattice's own buffer)
 template <typename R>
+// boundaryOnly: only the boundary pass (a force pass ahead of the FIRE pass
+// that keeps only Q and v, which computes the other sites' force itself)
 static int run_pass(qm_lattice* L, const qm_ldg* f, int mode, const FireConst& fire, bool gated = false,
-                    double* partials = nullptr) {
+                    double* partials = nullptr, bool storeF = false, bool boundaryOnly = false) {
     choose_zchunk(L, mode, terms_of(f));
     MarchArgs<R> ma{};
     BoundaryArgs<R> ba{};
@@ -1006,13 +1008,16 @@ static int run_pass(qm_lattice* L, const qm_ldg* f, int mode, const FireConst& f
     }
     ma.gate = gated ? 1 : 0;
     ba.gate = ma.gate;
+    ma.storeF = storeF ? 1 : 0;
     const int terms = terms_of(f);
     hipEvent_t ev[3];
     if (g_timing.on && mode == kFire) {
         for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
         HIP_TRY(hipEventRecord(ev[0], L->stream));
     }
-    if (is_stream_tile(tile))
+    if (boundaryOnly)
+        ;
+    else if (is_stream_tile(tile))
         HIP_TRY(launch_fire_stream<R>(ma, terms, tile, L->stream));
     else
         HIP_TRY(launch_march<R>(ma, mode, terms, tile, L->stream));
@@ -1451,18 +1456,25 @@ static int fire_drive(qm_fire* m, qm_ldg* f, qm_lattice* L, int maxIter, bool to
     if (int rc = push_ctrl(L)) return rc;
     FireConst fp = fire_const(m, L);
     if (!tolerance) fp.fc = -1.0;  // never converged: exactly maxIter - iterations steps
-    // sim->computeForces() at the start of minimize (energyMinimizerFIRE.cpp:157)
-    int rc = L->precision == QM_FP64 ? run_pass<double>(L, f, kForce, fp, false, partials)
-                                     : run_pass<float>(L, f, kForce, fp, false, partials);
+    // sim->computeForces() at the start of minimize (energyMinimizerFIRE.cpp:157).
+    // The FIRE pass that keeps only Q and v computes the force of the sites it
+    // finishes itself and reads only that of the boundary pass's sites.
+    const bool streamFire = is_stream_tile(active_tile(L, kFire));
+    int rc = L->precision == QM_FP64 ? run_pass<double>(L, f, kForce, fp, false, partials, false, streamFire)
+                                     : run_pass<float>(L, f, kForce, fp, false, partials, false, streamFire);
     if (rc) return rc;
     const int batch = L->n >= (1 << 22) ? 16 : 64;
+    // without a tolerance the run's last pass is known: it stores F as well
+    bool lastStoresF = false;
     while (true) {
         int todo = std::min(batch, maxIter - L->hCtrl.iterations);
         if (todo <= 0) todo = 1;  // at least one iteration per minimize call
         for (int k = 0; k < todo; ++k) {
-            rc = L->precision == QM_FP64 ? run_pass<double>(L, f, kFire, fp, false, partials)
-                                         : run_pass<float>(L, f, kFire, fp, false, partials);
+            const bool last = !tolerance && L->hCtrl.iterations + k + 1 == maxIter;
+            rc = L->precision == QM_FP64 ? run_pass<double>(L, f, kFire, fp, false, partials, last)
+                                         : run_pass<float>(L, f, kFire, fp, false, partials, last);
             if (rc) return rc;
+            lastStoresF = last;
         }
         if ((rc = sync_ctrl_to_host(L))) return rc;
         if (L->hCtrl.done) break;
@@ -1484,8 +1496,9 @@ static int fire_drive(qm_fire* m, qm_ldg* f, qm_lattice* L, int maxIter, bool to
                           hipMemcpyDeviceToHost));
     }
     // The pass that keeps only Q and v leaves F stale on sites it finishes
-    // itself: recompute F of the final state, which the caller may read.
-    if (is_stream_tile(active_tile(L, kFire))) {
+    // itself, unless it was the run's known last pass: recompute F of the final
+    // state, which the caller may read.
+    if (streamFire && !(lastStoresF && h.iterations == maxIter && !h.nanFlag)) {
         rc = L->precision == QM_FP64 ? run_pass<double>(L, f, kForce, fp, false, partials)
                                      : run_pass<float>(L, f, kForce, fp, false, partials);
         if (rc) return rc;

@@ -116,6 +116,7 @@ struct MarchArgs {
     const R* W;         // composed bulk operator weights (device buffer, read with scalar loads)
     const double* hvar; // spatially varying H, planes hx, hy, hz of n (null: none)
     double hvarP;       // deltaChi mu0
+    int storeF;         // FIRE pass that keeps only Q and v: store F of the sites it finishes as well
 };
 
 template <typename R>

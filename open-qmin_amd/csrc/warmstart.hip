@@ -7,11 +7,11 @@
 //  * conv_fwd_kernel: 3x3x3 dilated periodic convolution as an implicit GEMM,
 //    D[site][cout] = sum_{tap, cin} X[site + dil*tap][cin] Wt[tap][cout][cin],
 //    on the matrix cores: a wave owns 64 consecutive sites and all output
-//    channels; with 32 or 64 input channels each lane feeds 8 consecutive
-//    input channels of one neighbour site to v_mfma_f32_16x16x32_bf16
-//    (activations are channels-last, so that is one 16-byte load), with 16
-//    input channels 4 of them to v_mfma_f32_16x16x16_bf16; the tap's weights
-//    come from LDS.
+//    channels; with 64 input channels each lane feeds 8 consecutive input
+//    channels of one neighbour site to v_mfma_f32_16x16x32_bf16 (activations
+//    are channels-last, so that is one 16-byte load), with 16 or 32 input
+//    channels 4 of them to v_mfma_f32_16x16x16_bf16; the tap's weights come
+//    from LDS.
 //    Optional ReLU on the input as it is read, bias, and an output mask (the
 //    ReLU derivative of the previous layer when the kernel runs backwards).
 //  * The input gradient is the same kernel with the taps mirrored and the
@@ -81,9 +81,11 @@ constexpr int kFwdMT = 4;   // 16-site row tiles per wave in the forward kernel
 // against bank conflicts), double-buffered across taps.
 template <int NT, int KT>
 __global__ void __launch_bounds__(kWsThreads) conv_fwd_kernel(ConvArgs a) {
-    // K32: 32 input channels per matrix instruction; LDS rows padded by 8 BF16
-    // so that every lane's 16-byte weight fragment stays 16-byte aligned
-    constexpr bool K32 = KT % 2 == 0;
+    // K32: 32 input channels per matrix instruction (for 64 input channels; with
+    // 32, one such step per tap ran 9% slower at 128^3 than two 16-channel
+    // ones); LDS rows padded by 8 BF16 so that every lane's 16-byte weight
+    // fragment stays 16-byte aligned
+    constexpr bool K32 = KT == 4;
     constexpr int CO = NT * 16, CI = KT * 16, RS = CI + (K32 ? 8 : 4);   // LDS row stride in BF16
     __shared__ __align__(16) uint2 sW[2][CO * RS / 4];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;

@@ -88,6 +88,28 @@ def test_stream_fire_continues_and_moves_objects():
     np.testing.assert_allclose(g.v(), o.v(), rtol=0, atol=1e-11)
 
 
+@pytest.mark.parametrize("tile", [4, 5])
+def test_stream_run_iterations_stores_the_final_force(tile):
+    """run_iterations has no tolerance, so its last pass is known and stores F
+    itself (no separate force pass at the end): Q, the mixed velocity and F left
+    in the lattice are the oracle's after the same iterations, after a second
+    call too."""
+    print("stream run_iterations: final force, tile %d" % tile, flush=True)
+    _, g, f = build_pair((37, 21, 13), MULTI, 1, True)
+    g.set_tile(tile)
+    m = qmin.FIRE(max_iter=10 ** 6)
+    done = 0
+    for n in (17, 9):
+        st = m.run_iterations(f, g, n)
+        done += n
+        assert not st.nanDetected and st.iterations == done
+        o, _, _ = build_pair((37, 21, 13), MULTI, 1, True)   # the oracle runs from the start in one call
+        o.fire(max_iter=done, fc=0.0)
+        np.testing.assert_allclose(g.q(), o.q(), rtol=0, atol=1e-11)
+        np.testing.assert_allclose(g.v(), o.v(), rtol=0, atol=1e-11)
+        np.testing.assert_allclose(g.f(), o.f(), rtol=0, atol=1e-11)
+
+
 def test_stream_tile_with_l6_uses_the_fused_pass():
     """The streaming pass has no L6 form: with L6 a streaming tile setting
     leaves FIRE on the fused pass, which the oracle still matches."""
