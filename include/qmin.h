@@ -57,6 +57,13 @@ int qm_device_count(int* count);
 int qm_lattice_create(int lx, int ly, int lz, int precision, qm_lattice** out);
 int qm_lattice_destroy(qm_lattice* lat);
 int qm_lattice_dims(const qm_lattice* lat, int* lx, int* ly, int* lz, int* precision);
+/* the HIP stream (hipStream_t) the lattice's work runs on from now on, e.g. the
+ * caller's framework stream; NULL returns to the lattice's own stream.  It must
+ * have the default priority (QM_EINVAL otherwise); work queued on the previous
+ * stream completes first, and calls still return once their results are on the
+ * host.  No counterpart in the reference, whose GPU path runs on the default
+ * stream throughout. */
+int qm_lattice_set_stream(qm_lattice* lat, void* stream);
 /* z extent each work-group of the fused lattice pass marches through (>= 4);
  * 0 (default) sizes it so that the whole grid is resident at once.  A tuning
  * knob of this implementation; it has no counterpart in the reference. */

@@ -680,7 +680,7 @@ __global__ void materialize_velocity_kernel(StateSet<R> s0, StateSet<R> s1, long
 // derivatives there (central; one-sided towards the liquid crystal on an axis
 // with an object neighbour, the upper neighbour taking precedence).
 template <typename R>
-__device__ __forceinline__ void site_energy(const R* q, const int8_t* types, const DevObject* objects, int lx, int ly,
+__device__ __forceinline__ void site_energy(const R* q, const SiteType* types, const DevObject* objects, int lx, int ly,
                                             int lz, const EnergyConst& ec, long long i, double* parts,
                                             R (*dv)[5]) {
     const long long n = (long long)lx * ly * lz;
@@ -735,7 +735,7 @@ __device__ __forceinline__ void site_energy(const R* q, const int8_t* types, con
 
 // Energy: per-block partial sums of the five components over t <= 0 sites.
 template <typename R>
-__global__ void __launch_bounds__(kReduceThreads) energy_kernel(const R* q, const int8_t* types,
+__global__ void __launch_bounds__(kReduceThreads) energy_kernel(const R* q, const SiteType* types,
                                                                 const DevObject* objects, int lx, int ly, int lz,
                                                                 EnergyConst ec, double* partials) {
     __shared__ double sRed[kReduceThreads / 64][5];
@@ -770,7 +770,7 @@ __global__ void __launch_bounds__(kReduceThreads) energy_kernel(const R* q, cons
 // sites, else phase, fields, anchoring and distortion, added in the order of
 // computeEnergyCPU (landauDeGennesLC.cpp:204-353).
 template <typename R>
-__global__ void __launch_bounds__(kReduceThreads) energy_density_kernel(const R* q, const int8_t* types,
+__global__ void __launch_bounds__(kReduceThreads) energy_density_kernel(const R* q, const SiteType* types,
                                                                         const DevObject* objects, int lx, int ly,
                                                                         int lz, EnergyConst ec, double* out) {
     const long long n = (long long)lx * ly * lz;
@@ -793,7 +793,7 @@ __global__ void __launch_bounds__(kReduceThreads) energy_density_kernel(const R*
 // (+-1 per axis towards an object neighbour, the upper one taking precedence).
 // out[3 k + 0..2] is site k's contribution; the host adds them in site order.
 template <typename R>
-__global__ void object_force_kernel(const R* q, const int8_t* types, const DevObject* objects, int lx, int ly, int lz,
+__global__ void object_force_kernel(const R* q, const SiteType* types, const DevObject* objects, int lx, int ly, int lz,
                                     EnergyConst ec, double L1, const int64_t* sites, long long m, double* out) {
     for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < m; k += (long long)gridDim.x * blockDim.x) {
         const long long i = sites[k];
@@ -858,7 +858,7 @@ __global__ void scatter_q_kernel(R* q, long long n, const int64_t* dst, long lon
 // qTensorLatticeModel::computeDefectMeasures (qTensorLatticeModel.cpp:95-133) with
 // eigenvaluesOfQ (qTensorFunctions.h:215-241)
 template <typename R>
-__global__ void defect_kernel(const R* q, const int8_t* types, long long n, int kind, double* out) {
+__global__ void defect_kernel(const R* q, const SiteType* types, long long n, int kind, double* out) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
         if (types[i] > 0) {
             out[i] = 0.0;
@@ -1007,7 +1007,7 @@ void launch_boundary(const BoundaryArgs<R>& a, int mode, int terms, hipStream_t 
 }
 
 template <typename R>
-void launch_object_force(const R* q, const int8_t* types, const DevObject* objects, int lx, int ly, int lz,
+void launch_object_force(const R* q, const SiteType* types, const DevObject* objects, int lx, int ly, int lz,
                          const EnergyConst& ec, double L1, const int64_t* sites, long long m, double* out,
                          hipStream_t s) {
     const long long blocks = std::max<long long>(1, std::min<long long>((m + 127) / 128, 4096));
@@ -1050,7 +1050,7 @@ void launch_materialize(StateSet<R> s0, StateSet<R> s1, long long n, Ctrl* ctrl,
 }
 
 template <typename R>
-int launch_energy(const R* q, const int8_t* types, const DevObject* objects, int lx, int ly, int lz,
+int launch_energy(const R* q, const SiteType* types, const DevObject* objects, int lx, int ly, int lz,
                   const EnergyConst& ec, double* partials, hipStream_t s) {
     long long n = (long long)lx * ly * lz;
     long long blocks = (n + kReduceThreads - 1) / kReduceThreads;
@@ -1061,7 +1061,7 @@ int launch_energy(const R* q, const int8_t* types, const DevObject* objects, int
 }
 
 template <typename R>
-void launch_energy_density(const R* q, const int8_t* types, const DevObject* objects, int lx, int ly, int lz,
+void launch_energy_density(const R* q, const SiteType* types, const DevObject* objects, int lx, int ly, int lz,
                            const EnergyConst& ec, double* out, hipStream_t s) {
     const long long n = (long long)lx * ly * lz;
     long long blocks = (n + kReduceThreads - 1) / kReduceThreads;
@@ -1071,7 +1071,7 @@ void launch_energy_density(const R* q, const int8_t* types, const DevObject* obj
 }
 
 template <typename R>
-void launch_defects(const R* q, const int8_t* types, long long n, int kind, double* out, hipStream_t s) {
+void launch_defects(const R* q, const SiteType* types, long long n, int kind, double* out, hipStream_t s) {
     long long blocks = (n + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL((defect_kernel<R>), dim3(blocks), dim3(256), 0, s, q, types, n, kind, out);
@@ -1081,15 +1081,15 @@ void launch_defects(const R* q, const int8_t* types, long long n, int kind, doub
     template hipError_t launch_march<R>(const MarchArgs<R>&, int, int, int, hipStream_t);                                   \
     template void launch_boundary<R>(const BoundaryArgs<R>&, int, int, hipStream_t);                             \
     template void launch_materialize<R>(StateSet<R>, StateSet<R>, long long, Ctrl*, hipStream_t);                \
-    template int launch_energy<R>(const R*, const int8_t*, const DevObject*, int, int, int, const EnergyConst&,   \
+    template int launch_energy<R>(const R*, const SiteType*, const DevObject*, int, int, int, const EnergyConst&,   \
                                   double*, hipStream_t);                                                        \
-    template void launch_defects<R>(const R*, const int8_t*, long long, int, double*, hipStream_t);             \
-    template void launch_energy_density<R>(const R*, const int8_t*, const DevObject*, int, int, int,              \
+    template void launch_defects<R>(const R*, const SiteType*, long long, int, double*, hipStream_t);             \
+    template void launch_energy_density<R>(const R*, const SiteType*, const DevObject*, int, int, int,              \
                                            const EnergyConst&, double*, hipStream_t);                            \
     template int march_blocks_per_cu<R>(int, int, int);                                                          \
     template hipError_t launch_march_batch<R>(const MarchArgs<R>*, const int2*, int, int, int, int, hipStream_t);  \
     template hipError_t launch_boundary_batch<R>(const BoundaryArgs<R>*, const int2*, int, int, int, hipStream_t);  \
-    template void launch_object_force<R>(const R*, const int8_t*, const DevObject*, int, int, int, const EnergyConst&, \
+    template void launch_object_force<R>(const R*, const SiteType*, const DevObject*, int, int, int, const EnergyConst&, \
                                          double, const int64_t*, long long, double*, hipStream_t);                 \
     template void launch_move_q<R>(R*, long long, const int64_t*, const int64_t*, long long, R*, hipStream_t);
 

@@ -36,15 +36,15 @@ int stream_blocks_per_cu(int terms, int tile);
 template <typename R>
 void launch_materialize(StateSet<R> s0, StateSet<R> s1, long long n, Ctrl* ctrl, hipStream_t s);
 template <typename R>
-int launch_energy(const R* q, const int8_t* types, const DevObject* objects, int lx, int ly, int lz,
+int launch_energy(const R* q, const SiteType* types, const DevObject* objects, int lx, int ly, int lz,
                   const EnergyConst& ec, double* partials, hipStream_t s);
 template <typename R>
-void launch_defects(const R* q, const int8_t* types, long long n, int kind, double* out, hipStream_t s);
+void launch_defects(const R* q, const SiteType* types, long long n, int kind, double* out, hipStream_t s);
 template <typename R>
-void launch_energy_density(const R* q, const int8_t* types, const DevObject* objects, int lx, int ly, int lz,
+void launch_energy_density(const R* q, const SiteType* types, const DevObject* objects, int lx, int ly, int lz,
                            const EnergyConst& ec, double* out, hipStream_t s);
 template <typename R>
-void launch_object_force(const R* q, const int8_t* types, const DevObject* objects, int lx, int ly, int lz,
+void launch_object_force(const R* q, const SiteType* types, const DevObject* objects, int lx, int ly, int lz,
                          const EnergyConst& ec, double L1, const int64_t* sites, long long m, double* out,
                          hipStream_t s);
 template <typename R>
@@ -124,7 +124,8 @@ static int refresh_types(qm_lattice* L) {
     if (!L->typesDirty) return QM_OK;
     const long long n = L->n;
     const int lx = L->lx, ly = L->ly, lz = L->lz;
-    std::vector<int8_t> t8(n), cls(n);
+    std::vector<SiteType> t8(n);
+    std::vector<int8_t> cls(n);
     std::vector<int32_t> sites;
     int64_t active = 0;
     auto T = [&](int x, int y, int z) { return L->types[site_of(L, wrapc(x, lx), wrapc(y, ly), wrapc(z, lz))]; };
@@ -133,7 +134,7 @@ static int refresh_types(qm_lattice* L) {
             for (int x = 0; x < lx; ++x) {
                 const int64_t i = site_of(L, x, y, z);
                 const int t = L->types[i];
-                t8[i] = (int8_t)t;
+                t8[i] = (SiteType)t;
                 int c = t > 0 ? 1 : (t < 0 ? -1 : 0);
                 if (t == 0) {
                     static const int dir[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
@@ -152,7 +153,7 @@ static int refresh_types(qm_lattice* L) {
                 if (c < 0) sites.push_back((int32_t)i);
                 active += t <= 0;
             }
-    HIP_TRY(copy_on(L, L->dTypes, t8.data(), n, hipMemcpyHostToDevice));
+    HIP_TRY(copy_on(L, L->dTypes, t8.data(), n * sizeof(SiteType), hipMemcpyHostToDevice));
     HIP_TRY(copy_on(L, L->dClass, cls.data(), n, hipMemcpyHostToDevice));
     std::vector<DevObject> objs(kMaxObjects);
     for (size_t j = 0; j < L->objects.size(); ++j) objs[j] = {L->objects[j].type, 0, L->objects[j].W, L->objects[j].S0};
@@ -233,8 +234,9 @@ int qm_lattice_create(int lx, int ly, int lz, int precision, qm_lattice** out) {
     L->realSize = precision == QM_FP64 ? 8 : 4;
     L->types.assign(n, 0);
     HIP_TRY(hipGetDevice(&L->device));
-    HIP_TRY(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking));
-    HIP_TRY(hipMalloc(&L->dTypes, n));
+    HIP_TRY(hipStreamCreateWithFlags(&L->ownStream, hipStreamNonBlocking));
+    L->stream = L->ownStream;
+    HIP_TRY(hipMalloc(&L->dTypes, n * sizeof(SiteType)));
     HIP_TRY(hipMalloc(&L->dClass, n));
     HIP_TRY(hipMalloc(&L->dW, sizeof(double) * 2 * kFastWeights));
     HIP_TRY(hipMalloc(&L->dObjects, sizeof(DevObject) * kMaxObjects));
@@ -265,7 +267,7 @@ int qm_lattice_destroy(qm_lattice* L) {
     if (L->dPartials) (void)hipFree(L->dPartials);
     if (L->dTrace) (void)hipFree(L->dTrace);
     if (L->dHField) (void)hipFree(L->dHField);
-    (void)hipStreamDestroy(L->stream);
+    (void)hipStreamDestroy(L->ownStream);
     delete L;
     return QM_OK;
 }
@@ -355,6 +357,22 @@ int qm_lattice_set_tile(qm_lattice* L, int tile) {
     return QM_OK;
 }
 
+int qm_lattice_set_stream(qm_lattice* L, void* stream) {
+    if (!L) return fail(QM_EINVAL, "null lattice");
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : L->ownStream;
+    if (s != L->ownStream) {
+        // the lattice's own stream was created with the default priority
+        int prio = 0, own = 0;
+        HIP_TRY(hipStreamGetPriority(s, &prio));
+        HIP_TRY(hipStreamGetPriority(L->ownStream, &own));
+        if (prio != own) return fail(QM_EINVAL, "the stream must have the default priority");
+    }
+    // work queued so far finishes on the stream it was queued on
+    HIP_TRY(hipStreamSynchronize(L->stream));
+    L->stream = s;
+    return QM_OK;
+}
+
 int qm_lattice_set_zchunk(qm_lattice* L, int zchunk) {
     if (!L || zchunk < 0) return fail(QM_EINVAL, "bad argument");
     if (zchunk > 0 && zchunk < 4) return fail(QM_EINVAL, "z chunk must be 0 (automatic) or at least 4");
@@ -439,7 +457,7 @@ int qm_lattice_add_object(qm_lattice* L, const int32_t* xyz, int64_t n, const do
                           double S0, int* obj) {
     if (!L || (n > 0 && (!xyz || !qb))) return fail(QM_EINVAL, "bad argument");
     if (btype != QM_HOMEOTROPIC && btype != QM_DEGENERATE_PLANAR) return fail(QM_EINVAL, "unknown boundary type");
-    if ((int)L->objects.size() >= kMaxObjects) return fail(QM_EINVAL, "at most 127 objects per lattice");
+    if ((int)L->objects.size() >= kMaxObjects) return fail(QM_EINVAL, "at most 32767 objects per lattice");
     std::vector<double> q(5 * L->n);
     if (int rc = download_field(L, QM_FIELD_Q, q.data())) return rc;
     std::vector<int64_t> sites;

@@ -35,7 +35,8 @@ struct Tile {
     static_assert(Halo <= Threads, "one halo site per thread at most");
 };
 
-constexpr int kMaxObjects = 127;        // types are int8 on the device
+using SiteType = int16_t;                // site types on the device (reference encoding)
+constexpr int kMaxObjects = 32767;      // object ids a SiteType holds
 constexpr int kBoundaryThreads = 128;
 constexpr int kReduceThreads = 256;
 constexpr int kMaxEnergyBlocks = 4096;
@@ -122,7 +123,7 @@ struct MarchArgs {
 template <typename R>
 struct BoundaryArgs {
     StateSet<R> set[2];
-    const int8_t* types;
+    const SiteType* types;
     const DevObject* objects;
     const int32_t* sites;   // compacted list of sites finished here
     int gate;

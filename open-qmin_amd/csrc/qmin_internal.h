@@ -28,10 +28,11 @@ struct qm_lattice {
     long long n = 0;
     size_t realSize = 8;
     int device = 0;                  // the GPU that holds the lattice (current at creation)
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;    // the stream all of the lattice's work runs on
+    hipStream_t ownStream = nullptr; // the lattice's own (stream unless the caller set one)
     std::vector<int32_t> types;      // host copy, reference encoding
     std::vector<HostObject> objects;
-    int8_t* dTypes = nullptr;
+    qmin::SiteType* dTypes = nullptr;
     int8_t* dClass = nullptr;             // 0 composed bulk formula, > 0 object, < 0 boundary pass
     void* dW = nullptr;                   // composed bulk operator weights (Real)
     DevObject* dObjects = nullptr;

@@ -402,7 +402,7 @@ __global__ void __launch_bounds__(kWsThreads) mse_grad_kernel(const float* y, in
 // channels of liquid-crystal sites, F being the force computer's force on the
 // predicted state (minus the energy gradient, shaped by the metric).
 template <typename R>
-__global__ void __launch_bounds__(kWsThreads) energy_grad_kernel(const R* f, const int8_t* types, long long n,
+__global__ void __launch_bounds__(kWsThreads) energy_grad_kernel(const R* f, const SiteType* types, long long n,
                                                                int cy, float scale, uint16_t* g) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
         if (types[i] > 0) continue;
@@ -418,7 +418,7 @@ __global__ void __launch_bounds__(kWsThreads) energy_grad_kernel(const R* f, con
 //  8   planar flag, 9 log(W) / 3 (object sites),
 //  10-14 far-field tensor d d - I/3 (every site), 15 zero.
 template <typename R>
-__global__ void features_kernel(const R* q, const int8_t* types, const DevObject* objects, long long n, double dx,
+__global__ void features_kernel(const R* q, const SiteType* types, const DevObject* objects, long long n, double dx,
                                 double dy, double dz, uint16_t* out) {
     const float far[5] = {float(dx * dx - 1.0 / 3.0), float(dx * dy), float(dx * dz), float(dy * dy - 1.0 / 3.0),
                           float(dy * dz)};
@@ -456,8 +456,13 @@ __global__ void features_kernel(const R* q, const int8_t* types, const DevObject
 }
 
 // Q of liquid-crystal sites from the first five channels of y; velocities zero.
+__global__ void types_to_i8_kernel(const SiteType* types, long long n, int8_t* out) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        out[i] = int8_t(types[i] > 127 ? 127 : types[i]);
+}
+
 template <typename R>
-__global__ void apply_kernel(const float* y, int cy, const int8_t* types, long long n, R* q, R* u, Off5 off) {
+__global__ void apply_kernel(const float* y, int cy, const SiteType* types, long long n, R* q, R* u, Off5 off) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
         if (types[i] <= 0) {
 #pragma unroll
@@ -663,10 +668,14 @@ int qm_ws_features(qm_lattice* L, double dx, double dy, double dz, void* out) {
     return QM_OK;
 }
 
-// the device site types (int8, reference encoding clamped to the object index)
+// the device site types as int8 (reference encoding, object ids above 127 clamped
+// to 127: the network's inputs and loss only tell objects from the rest)
 int qm_ws_types(qm_lattice* L, void* out) {
     if (!L || !out) return fail(QM_EINVAL, "bad argument");
-    HIP_TRY(copy_on(L, out, L->dTypes, L->n, hipMemcpyDeviceToDevice));
+    hipLaunchKernelGGL(types_to_i8_kernel, dim3(grid_for(L->n)), dim3(kWsThreads), 0, L->stream, L->dTypes, L->n,
+                       static_cast<int8_t*>(out));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(L->stream));
     return QM_OK;
 }
 

@@ -105,6 +105,7 @@ _API = {
     "qm_fire_minimize": [_VP, _VP, _VP],
     "qm_fire_run_iterations": [_VP, _VP, _VP, _i],
     "qm_fire_run_iterations_noalloc": [_VP, _VP, _VP, _i],
+    "qm_lattice_set_stream": [_VP, _VP],
     "qm_fire_get_state": [_VP, ctypes.POINTER(FireState)],
     "qm_fire_set_trace": [_VP, _i],
     "qm_fire_get_trace": [_VP, _D, _i, ctypes.POINTER(ctypes.c_int)],
@@ -196,6 +197,12 @@ class Lattice:
 
     def set_zchunk(self, zchunk):
         _check(self.lib.qm_lattice_set_zchunk(self.h, zchunk))
+
+    def set_stream(self, stream):
+        """Run the lattice's work on a caller's HIP stream (an integer handle,
+        e.g. torch.cuda.Stream().cuda_stream, default priority); None returns
+        to the lattice's own stream."""
+        _check(self.lib.qm_lattice_set_stream(self.h, ctypes.c_void_p(stream) if stream else None))
 
     def set_hfield(self, h3):
         """Spatially varying H field, [n][3] in site order (None removes it)."""

@@ -618,7 +618,6 @@ void orc_director_init(orc_system* s, const double* n4) {
 // multirankSimulation::createMultirankBoundaryObject (multirankSimulationBoundaries.cpp:512-543)
 // followed by cubicLattice::createBoundaryObject (cubicLattice.cpp:267-318).
 int orc_add_object(orc_system* s, const int32_t* xyz, int n, const double* qb, int btype, double W, double S0) {
-    if (s->objects.size() >= 127) return -1;
     std::vector<int64_t> sites;
     sites.reserve(n);
     for (int ii = 0; ii < n; ++ii) {

@@ -272,3 +272,64 @@ def test_one_ensemble_lattice_with_nan_leaves_the_others_running():
         alone.sphere((10.0, 9.0, 8.0), 4.0, qmin.HOMEOTROPIC, 5.3, S0)
         st = qmin.FIRE(dt=5e-4, dt_max=2e-2, fc=1e-3, max_iter=3000).minimize(f, alone)
         assert st.iterations == its[k]
+
+
+@pytest.mark.gpu
+def test_more_than_127_objects_match_oracle():
+    """Object ids above 127 (site types are 16-bit on the device; the reference
+    keeps int types): 150 small spheres of both anchorings and several W on a
+    40^3 lattice.  Types, forces and energy against the oracle, then FIRE."""
+    import qmin
+    shape = (40, 40, 40)
+    o = po.OracleSystem(*shape)
+    g = qmin.Lattice(*shape)
+    o.random_init(21, S0)
+    g.random_init(21, S0)
+    rng = np.random.default_rng(150)
+    for k in range(150):
+        c = tuple(float(v) for v in rng.uniform(0, 40, 3))
+        bt, W = k % 2, float(rng.uniform(1, 10))
+        o.sphere(c, 1.3, bt, W, S0)
+        g.sphere(c, 1.3, bt, W, S0)
+    t = g.types()
+    assert np.array_equal(t, o.types())
+    assert t.max() > 127, t.max()
+    o.set_params(A, B, C, 2.32, 0.5, 0.3, multi=True, semantics=1)
+    f = qmin.LandauDeGennesLC(A, B, C, 2.32, 0.5, 0.3, semantics=1)
+    ref = o.forces()
+    got = f.compute_forces(g)
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    e_ref, _ = o.energy()
+    e_got, _ = f.compute_energy(g)
+    assert abs(e_got - e_ref) <= 1e-12 * abs(e_ref)
+    _, tr_ref = o.fire(max_iter=10)
+    m = qmin.FIRE(max_iter=10)
+    m.set_trace(16)
+    m.minimize(f, g)
+    np.testing.assert_allclose(m.trace()[:, 0], tr_ref[:, 0], rtol=1e-10)
+    np.testing.assert_allclose(g.q(), o.q(), rtol=0, atol=1e-11)
+
+
+@pytest.mark.gpu
+def test_lattice_on_a_caller_stream():
+    """qm_lattice_set_stream: the lattice's work runs on a caller's stream (here
+    torch's, default priority) and gives the same result as on its own; None
+    returns to the lattice's own stream."""
+    torch = pytest.importorskip("torch")
+    import qmin
+    res = []
+    for use in (False, True):
+        g = qmin.Lattice(24, 20, 16)
+        g.random_init(4, S0)
+        g.sphere((12.0, 10.0, 8.0), 4.0, qmin.HOMEOTROPIC, 5.3, S0)
+        s = torch.cuda.Stream()
+        if use:
+            g.set_stream(s.cuda_stream)
+        f = qmin.LandauDeGennesLC(A, B, C, 2.32, 0.5, 0.3, semantics=1)
+        st = qmin.FIRE(max_iter=25).minimize(f, g)
+        if use:
+            g.set_stream(None)
+            st2 = qmin.FIRE(max_iter=5).minimize(f, g)
+            assert st2.iterations == 5
+        res.append((st.forceMax, g.q()))
+    assert res[0][0] == res[1][0]
