@@ -252,7 +252,7 @@ def ensemble_throughput(qmin, count=1024, fc=1e-7, max_iter=10000, sequential_sa
     state with FP64 reductions): the tile autotuner picks the batched pass's
     shape (outside the timed region), then FIRE on all lattices at once, batched
     launches replayed from a hipGraph; the first lattices are also minimized one
-    by one for comparison.  dt_max is 40 dt: at the driver's 100 dt FIRE diverges on
+    by one for comparison.  dt_max is 40 dt: at openQmin's default 100 dt FIRE diverges on
     stiff planar-anchored spheres (the reference algorithm does too)."""
     from qmin import workloads
     A, B, C, S0 = workloads.phase_constants()

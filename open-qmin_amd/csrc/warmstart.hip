@@ -1,7 +1,7 @@
 // Learned warm start: a small fully-convolutional 3-D network maps a lattice's
 // geometry (site types, anchoring tensors and strengths, a chosen far-field
 // director) to an initial Q field, from which FIRE then minimizes.  Random-start
-// FIRE stays the fallback and the judge of the final state.
+// FIRE stays the fallback and decides the final state.
 //
 // Kernels (BF16 operands on the matrix cores, FP32 accumulation):
 //  * conv_fwd_kernel: 3x3x3 dilated periodic convolution as an implicit GEMM,

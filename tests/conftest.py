@@ -33,7 +33,7 @@ def example_directors(shape):
     in the reference's setInitialConditions.h (openQmin -z 3): director
     (x, x + y + cos z, 0) normalized as normalizeDirector does (inc/functions.h:
     193-205), S0 = 0.53 + 1e-10 (x + y + z).  math.cos is the C library's cos,
-    as in the reference's and the driver's builds."""
+    as in the reference's build and in openQmin-hip's."""
     import math
     import numpy as np
     lx, ly, lz = shape
