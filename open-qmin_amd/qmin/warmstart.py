@@ -311,6 +311,16 @@ def heldout_geometry(L, seed):
     return out
 
 
+def diagonal_start_specs(count, size=64, seed=8192):
+    """The ensemble workload's geometries at size^3 (as random_start_specs)
+    started from a uniform director along diagonal_director(): targets in the
+    far-field orientation predictions on periodic lattices use."""
+    specs = workloads.ensemble_specs(count, seed, edges=(size,))
+    for sp in specs:
+        sp["director"] = diagonal_director()
+    return specs
+
+
 def random_start_specs(count, size=64, seed=4096):
     """Training geometries as the ensemble workload's at size^3 (periodic, 0-3
     spheres of radius 4-16, homeotropic or degenerate planar, W log-uniform in

@@ -1,15 +1,17 @@
 """Train the warm-start network on converged lattices from the ensemble
 minimizer and measure what it buys on held-out geometries.
 
-  python open-qmin_amd/qmin/warmstart_experiment.py [--train 2048] [--train-walls 0]
+  python open-qmin_amd/qmin/warmstart_experiment.py [--train 2048] [--train-walls 0] [--train-diagonal 0]
          [--epochs 6] [--lr 5e-4] [--energy-weight 0.1] [--max-iter 6000]
          [--init weights.npz] [--weights out.npz] [--load weights.npz] [--skip-256]
 
 Training set: `--train` 64^3 periodic lattices as the ensemble workload's (0-3
 spheres of radius 4-16, homeotropic or degenerate planar, W log-uniform in
 [0.5, 20]) from seeded random-director starts (L1 = 2.32, L2 = 0.5, L3 = 0.3 as
-the evaluation), and `--train-walls` 64^3 lattices of warmstart.training_specs
-(half with z walls, uniform starts), minimized together in FP32 by the ensemble
+the evaluation), `--train-walls` 64^3 lattices of warmstart.training_specs
+(half with z walls, uniform starts) and `--train-diagonal` lattices of
+warmstart.diagonal_start_specs (the ensemble's geometries, uniform starts along
+a body diagonal), minimized together in FP32 by the ensemble
 minimizer to forceMax < 1e-7 (at most `--max-iter` FIRE iterations); the
 converged Q is the target, the principal axis of its mean bulk Q the far-field
 input.  Loss: mean squared error to the target plus `--energy-weight` times the
@@ -88,6 +90,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--train", type=int, default=2048)
     ap.add_argument("--train-walls", type=int, default=0)
+    ap.add_argument("--train-diagonal", type=int, default=0)
     ap.add_argument("--init", default="")
     ap.add_argument("--epochs", type=int, default=6)
     ap.add_argument("--lr", type=float, default=5e-4)
@@ -110,13 +113,16 @@ def main():
         specs = ws.random_start_specs(args.train, 64)
         if args.train_walls:
             specs += ws.training_specs(args.train_walls, 64)
+        if args.train_diagonal:
+            specs += ws.diagonal_start_specs(args.train_diagonal, 64)
         samples = ws.make_dataset(qmin, specs, force, max_iter=args.max_iter, log=log,
                                   keep_lattices=args.energy_weight > 0)
         t1 = time.perf_counter()
         hist = net.train(samples, epochs=args.epochs, lr=args.lr, log=log, force=force,
                          energy_weight=args.energy_weight)
         t2 = time.perf_counter()
-        result["training"] = {"lattices": args.train, "wall_lattices": args.train_walls, "size": 64,
+        result["training"] = {"lattices": args.train, "wall_lattices": args.train_walls,
+                              "diagonal_lattices": args.train_diagonal, "size": 64,
                               "init": os.path.basename(args.init), "data_seconds": t1 - t0,
                               "train_seconds": t2 - t1, "epochs": args.epochs, "energy_weight": args.energy_weight,
                               "mse_per_epoch": hist}
