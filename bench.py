@@ -124,6 +124,9 @@ def tolerance_chain(qmin, L, mode, fcs, switch_fc=None, max_iter=400000, ring=Fa
             from qmin import defects
             c = L / 2.0
             r["defect_ring"] = defects.ring_locus(lat, (c, c, c), 20.0 * L / 256.0)
+            # the reference's forceMax is |F|_2 / N; the largest force on any one site, for comparison
+            import numpy as np
+            r["max_site_force"] = float(np.sqrt((lat.f() ** 2).sum(axis=1)).max())
         print("time to forceMax<%g (%s): %d iterations, %.2f s" % (fc, mode, st.iterations, elapsed),
               file=sys.stderr, flush=True)
         out.append(r)
@@ -235,6 +238,7 @@ def compare_modes(qmin, L, fc=1e-8, tight=(1e-10, 1e-11), switch_fc=MIXED_SWITCH
     mixed = time_to_tolerance(qmin, L, "mixed", fc)
     first = _pair(fc, fp64[0], mixed)
     first["defect_ring"] = fp64[0].pop("defect_ring")
+    first["fp64_max_site_force"] = fp64[0].pop("max_site_force")
     tight_out = {}
     if tight:
         mixed_tight = tolerance_chain(qmin, L, "mixed", tuple(tight), switch_fc)
