@@ -150,3 +150,24 @@ def test_walls_sphere_256_full_size():
     m.minimize(f, g)
     np.testing.assert_allclose(m.trace()[:, 0], tr_ref[:, 0], rtol=1e-10)
     np.testing.assert_allclose(g.q(), o.q(), rtol=0, atol=1e-11)
+
+
+def test_fire_mixed_hands_over_when_fp32_stalls():
+    """qmin.mixed.fire_mixed with no switch tolerance: FP32 runs until its
+    forceMax stops falling (its rounding floor), then FP64 polishes to 1e-11.
+    On the 48^3 walls-and-sphere scene the FP32 phase ends above 1e-11 and the
+    converged energy is within 1e-9 relative of an all-FP64 run's."""
+    from qmin import mixed
+    from bench import build_workload, fire_params
+    print("mixed FIRE with a stall hand-over, 48^3", flush=True)
+    g64, f = build_workload(qmin, 48, qmin.QM_FP64)
+    st64 = qmin.FIRE(fc=1e-11, max_iter=400000, **fire_params()).minimize(f, g64)
+    assert st64.converged
+    e64, _ = f.compute_energy(g64)
+    gmx, _ = build_workload(qmin, 48, qmin.QM_FP32)
+    m, st, fp32_its = mixed.fire_mixed(f, gmx, 1e-11, chunk=1000, max_iter=400000, **fire_params())
+    assert st.converged and st.forceMax < 1e-11
+    assert 0 < fp32_its < st.iterations
+    assert gmx.precision == qmin.QM_FP64
+    emx, _ = f.compute_energy(gmx)
+    assert abs(emx - e64) <= 1e-9 * abs(e64), (emx, e64)
