@@ -11,7 +11,7 @@ OBJ = os.path.join(HERE, "build", "obj")
 LIB = os.path.join(HERE, "lib", "libqmin.so")
 DRIVER = os.path.join(HERE, "lib", "openQmin-hip")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
-SOURCES = ["ldg_kernels.hip", "fire_stream.hip", "qmin_capi.hip", "minimizers.hip", "ensemble.hip", "warmstart.hip"]
+SOURCES = ["ldg_kernels.hip", "step_kernels.hip", "fire_stream.hip", "qmin_capi.hip", "minimizers.hip", "ensemble.hip", "warmstart.hip"]
 # host code that must match the reference's g++ build bit for bit
 HOST_SOURCES = ["host_init.cpp"]
 GXX = os.environ.get("CXX", "g++")

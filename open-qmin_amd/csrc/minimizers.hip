@@ -1,12 +1,14 @@
 // Adam, Nesterov accelerated gradient and online L-BFGS on the device.
 //
-// An iteration of each is the shared force pass (the fused lattice pass in
-// force mode plus the boundary pass) and streaming kernels over the 5 n
-// components of Q, F and the minimizer's own vectors (same plane layout as Q).
-// Every streaming kernel reduces the dot products it needs into per-work-group
-// partials; a one-work-group finalize kernel sums them in a fixed order and
-// finishes the minimizer's scalar logic in device memory (step coefficients,
-// forceMax, the stopping test).  (A separate kernel, not a last-work-group
+// An Adam or Nesterov iteration is one lattice pass (the fused and boundary
+// passes in mode kAdamStep / kNesterovStep, step_kernels.hip): each site's
+// force, then the minimizer's update of that site, written to the other state
+// set.  An L-BFGS iteration is the shared force pass (force mode) and
+// streaming kernels over the 5 n components of Q, F and the minimizer's own
+// vectors (same plane layout as Q).  Every pass reduces the dot products it
+// needs into per-work-group partials; a one-work-group finalize kernel sums
+// them in a fixed order and finishes the minimizer's scalar logic in device
+// memory (step coefficients, forceMax, the stopping test).  (A separate kernel, not a last-work-group
 // pattern: the device-wide fences that pattern needs write back each XCD's L2,
 // which the streaming kernel has just filled.)  A batch of iterations is
 // therefore queued without host round trips, every kernel after the stopping
@@ -89,99 +91,6 @@ struct Common {
     double fc, ndof;
     long long len;      // 5 n
 };
-
-// ---------------------------------------------------------------- Adam
-template <typename R>
-struct AdamArgs {
-    R* q;
-    const R* f;
-    R* m;
-    R* v;
-    double* scal;   // [0] beta1^t, [1] beta2^t
-    double b1, b2, eps, dt;
-    Common c;
-};
-
-// energyMinimizerAdam::adamStepCPU (energyMinimizerAdam.cpp:25-54): the
-// displacement uses epsilon where sqrt(vc) is 0.
-template <typename R>
-__global__ void __launch_bounds__(kVecThreads) adam_kernel(AdamArgs<R> A) {
-    Ctrl* ctrl = A.c.ctrl;
-    if (ctrl->done) return;
-    const double b1t = A.scal[0], b2t = A.scal[1];
-    const R b1 = R(A.b1), b2 = R(A.b2), b1m1 = R(A.b1 - 1), omb2 = R(1 - A.b2);
-    const R inv1 = R(1.0 / (1.0 - b1t)), inv2 = R(1.0 / (1.0 - b2t));
-    const R eps = R(A.eps), mdt = R(-A.dt);
-    double ff = 0;
-    const long long stride = (long long)gridDim.x * kVecThreads * kVecUnroll;
-    for (long long base = (long long)blockIdx.x * kVecThreads * kVecUnroll + threadIdx.x; base < A.c.len;
-         base += stride) {
-#pragma unroll
-        for (int u = 0; u < kVecUnroll; ++u) {
-            const long long i = base + u * kVecThreads;
-            if (i < A.c.len) {
-                const R f = A.f[i];
-                ff += double(f) * double(f);
-                const R m = b1 * A.m[i] + b1m1 * f;
-                const R v = b2 * A.v[i] + omb2 * (f * f);
-                const R mc = m * inv1;
-                const R vc = v * inv2;
-                R root = sqrt(vc);
-                if (root == R(0)) root = eps;
-                A.m[i] = m;
-                A.v[i] = v;
-                A.q[i] = A.q[i] + mdt * mc / root;
-            }
-        }
-    }
-    double part[1] = {ff};
-    write_partials<1>(part, A.c.partials);
-}
-
-// ---------------------------------------------------------------- Nesterov
-template <typename R>
-struct NesterovArgs {
-    R* q;
-    const R* f;
-    R* alt;
-    double* scal;   // [0] lambda, [1] mu
-    double dt;
-    int scheduled;
-    Common c;
-};
-
-// energyMinimizerNesterovAG::minimize + nesterovStepCPU
-// (energyMinimizerNesterovAG.cpp:70-120): lambda and the scheduled momentum
-// advance once per executed iteration.
-template <typename R>
-__global__ void __launch_bounds__(kVecThreads) nesterov_kernel(NesterovArgs<R> A) {
-    Ctrl* ctrl = A.c.ctrl;
-    if (ctrl->done) return;
-    const double oldLambda = A.scal[0];
-    const double lambda = 0.5 * (1.0 + sqrt(1.0 + 4.0 * oldLambda * oldLambda));
-    const double gamma = (1 - oldLambda) / lambda;
-    const double mud = A.scheduled ? 1. + gamma : A.scal[1];
-    const R mu = R(mud), dt = R(A.dt);
-    double ff = 0;
-    const long long stride = (long long)gridDim.x * kVecThreads * kVecUnroll;
-    for (long long base = (long long)blockIdx.x * kVecThreads * kVecUnroll + threadIdx.x; base < A.c.len;
-         base += stride) {
-#pragma unroll
-        for (int u = 0; u < kVecUnroll; ++u) {
-            const long long i = base + u * kVecThreads;
-            if (i < A.c.len) {
-                const R f = A.f[i];
-                ff += double(f) * double(f);
-                const R old = A.alt[i];
-                const R alt = A.q[i] + dt * f;
-                A.alt[i] = alt;
-                A.q[i] = alt + mu * (alt - old);
-            }
-        }
-    }
-    double part[1] = {ff};
-    write_partials<1>(part, A.c.partials);
-}
 
 // ---------------------------------------------------------------- online L-BFGS
 // Device scalars of the two-loop recursion, per memory slot t and loop index ii:
@@ -325,7 +234,9 @@ enum FinOp { kFinAdam, kFinNesterov, kFinAlpha, kFinBeta, kFinLbFinish, kFinLbSt
 
 struct FinArgs {
     int op;
-    int blocks;       // work-groups of the streaming kernel whose partials are summed
+    int blocks;       // partial-sum slots to sum
+    int stride;       // doubles per slot: kMaxDots for the streaming kernels, 3 for the lattice passes
+    int flip;         // the pass wrote the other state set: make it current
     int ii, t;        // two-loop index and memory slot
     LbLayout lay;
     double b1, b2;    // Adam
@@ -341,9 +252,11 @@ __global__ void __launch_bounds__(kVecThreads) vec_finalize_kernel(FinArgs A) {
     Ctrl* ctrl = A.c.ctrl;
     if (ctrl->done) return;
     double tot[kMaxDots] = {0, 0, 0, 0};
+    const int stride = A.stride;
     for (int j = threadIdx.x; j < A.blocks; j += kVecThreads) {
 #pragma unroll
-        for (int k = 0; k < kMaxDots; ++k) tot[k] += A.c.partials[kMaxDots * j + k];
+        for (int k = 0; k < kMaxDots; ++k)
+            if (k < stride) tot[k] += A.c.partials[(long long)stride * j + k];
     }
     block_sum<kMaxDots>(tot, red);
     if (threadIdx.x != 0) return;
@@ -355,6 +268,7 @@ __global__ void __launch_bounds__(kVecThreads) vec_finalize_kernel(FinArgs A) {
             scal[0] = scal[0] * A.b1;
             scal[1] = scal[1] * A.b2;
             finish_iteration(ctrl, sqrt(tot[0]) / A.c.ndof, A.c.fc, A.c.trace);
+            if (A.flip) ctrl->cur ^= 1;
             break;
         case kFinNesterov: {
             // the same lambda and momentum the step kernel computed from scal[0]
@@ -364,6 +278,7 @@ __global__ void __launch_bounds__(kVecThreads) vec_finalize_kernel(FinArgs A) {
             if (A.scheduled) scal[1] = 1. + gamma;
             scal[0] = lambda;
             finish_iteration(ctrl, sqrt(tot[0]) / A.c.ndof, A.c.fc, A.c.trace);
+            if (A.flip) ctrl->cur ^= 1;
             break;
         }
         case kFinAlpha: {
@@ -495,11 +410,15 @@ struct VecMinimizer {
         HIP_TRY(hipMemcpyAsync(dScal + offset, v, count * sizeof(double), hipMemcpyHostToDevice, L->stream));
         return QM_OK;
     }
-    // the finalize kernel for a streaming kernel of `blocks` work-groups
-    int finalize(const qm_lattice* L, FinArgs a, int blocks) const {
-        a.blocks = blocks;
+    // the finalize kernel for a streaming kernel of `blocks` work-groups, or
+    // (partials given) for a lattice pass of `blocks` slots of 3 doubles
+    int finalize(const qm_lattice* L, FinArgs a, long long blocks, double* partials = nullptr) const {
+        a.blocks = (int)blocks;
+        a.stride = partials ? 3 : kMaxDots;
+        a.flip = partials ? 1 : 0;
         a.scal = dScal;
         a.c = common(L);
+        if (partials) a.c.partials = partials;
         vec_finalize_kernel<<<1, kVecThreads, 0, L->stream>>>(a);
         HIP_TRY(hipGetLastError());
         return QM_OK;
@@ -572,29 +491,22 @@ struct qm_adam : VecMinimizer {
     double b1t = 0.9, b2t = 0.99;
 };
 
-template <typename R>
 static int adam_enqueue(qm_adam* m, qm_lattice* L, const qm_ldg* f) {
-    if (int rc = force_pass(L, f, true)) return rc;
-    const StateSet<R> st = state_set<R>(L, L->hCtrl.cur);
-    AdamArgs<R> a{};
-    a.q = st.q;
-    a.f = st.f;
-    a.m = rp<R>(m->vecs[0]);
-    a.v = rp<R>(m->vecs[1]);
-    a.scal = m->dScal;
-    a.b1 = m->b1;
-    a.b2 = m->b2;
-    a.eps = m->eps;
-    a.dt = m->dt;
-    a.c = m->common(L);
-    const int blocks = vec_blocks(a.c.len);
-    adam_kernel<R><<<blocks, kVecThreads, 0, L->stream>>>(a);
-    HIP_TRY(hipGetLastError());
+    StepArgs st{};
+    st.m = m->vecs[0];
+    st.v = m->vecs[1];
+    st.scal = m->dScal;
+    st.b1 = m->b1;
+    st.b2 = m->b2;
+    st.eps = m->eps;
+    st.dt = m->dt;
+    long long slots = 0;
+    if (int rc = step_pass(L, f, kAdamStep, st, &slots)) return rc;
     FinArgs fa{};
     fa.op = kFinAdam;
     fa.b1 = m->b1;
     fa.b2 = m->b2;
-    return m->finalize(L, fa, blocks);
+    return m->finalize(L, fa, slots, L->dPartials);
 }
 
 extern "C" {
@@ -640,11 +552,11 @@ int qm_adam_minimize(qm_adam* m, qm_ldg* f, qm_lattice* L) {
     }
     m->forceMax = 110.0;
     if (int rc = begin_run(*m, L, f, m->iterations >= m->maxIter)) return rc;
+    // the iterations alternate the state sets and leave U alone
+    if (int rc = share_velocity(L)) return rc;
     const double sc[2] = {m->b1t, m->b2t};
     if (int rc = m->set_scalars(L, sc, 2)) return rc;
-    auto enq = [&](int) {
-        return L->precision == QM_FP64 ? adam_enqueue<double>(m, L, f) : adam_enqueue<float>(m, L, f);
-    };
+    auto enq = [&](int) { return adam_enqueue(m, L, f); };
     int rc = drive(*m, L, enq);
     double back[2];
     if (int r2 = m->get_scalars(L, back, 2)) return r2;
@@ -671,25 +583,18 @@ struct qm_nesterov : VecMinimizer {
     int scheduled = 0;
 };
 
-template <typename R>
 static int nesterov_enqueue(qm_nesterov* m, qm_lattice* L, const qm_ldg* f) {
-    if (int rc = force_pass(L, f, true)) return rc;
-    const StateSet<R> st = state_set<R>(L, L->hCtrl.cur);
-    NesterovArgs<R> a{};
-    a.q = st.q;
-    a.f = st.f;
-    a.alt = rp<R>(m->vecs[0]);
-    a.scal = m->dScal;
-    a.dt = m->dt;
-    a.scheduled = m->scheduled;
-    a.c = m->common(L);
-    const int blocks = vec_blocks(a.c.len);
-    nesterov_kernel<R><<<blocks, kVecThreads, 0, L->stream>>>(a);
-    HIP_TRY(hipGetLastError());
+    StepArgs st{};
+    st.m = m->vecs[0];
+    st.scal = m->dScal;
+    st.dt = m->dt;
+    st.scheduled = m->scheduled;
+    long long slots = 0;
+    if (int rc = step_pass(L, f, kNesterovStep, st, &slots)) return rc;
     FinArgs fa{};
     fa.op = kFinNesterov;
     fa.scheduled = m->scheduled;
-    return m->finalize(L, fa, blocks);
+    return m->finalize(L, fa, slots, L->dPartials);
 }
 
 extern "C" {
@@ -733,6 +638,8 @@ int qm_nesterov_minimize(qm_nesterov* m, qm_ldg* f, qm_lattice* L) {
         m->mu = m->mu0;
     }
     if (int rc = begin_run(*m, L, f, m->iterations >= m->maxIter)) return rc;
+    // the iterations alternate the state sets and leave U alone
+    if (int rc = share_velocity(L)) return rc;
     if (rebind) {
         // initializeFromModel (energyMinimizerNesterovAG.cpp:31-37): the
         // alternate sequence starts as a copy of the positions
@@ -741,9 +648,7 @@ int qm_nesterov_minimize(qm_nesterov* m, qm_ldg* f, qm_lattice* L) {
     }
     const double sc[2] = {m->lambda, m->mu};
     if (int rc = m->set_scalars(L, sc, 2)) return rc;
-    auto enq = [&](int) {
-        return L->precision == QM_FP64 ? nesterov_enqueue<double>(m, L, f) : nesterov_enqueue<float>(m, L, f);
-    };
+    auto enq = [&](int) { return nesterov_enqueue(m, L, f); };
     int rc = drive(*m, L, enq);
     double back[2];
     if (int r2 = m->get_scalars(L, back, 2)) return r2;

@@ -34,6 +34,9 @@ hipError_t launch_fire_stream(const MarchArgs<R>& a, int terms, int tile, hipStr
 template <typename R>
 int stream_blocks_per_cu(int terms, int tile);
 template <typename R>
+hipError_t launch_step(const MarchArgs<R>& ma, const BoundaryArgs<R>& ba, int mode, int terms, int tile,
+                       hipStream_t s);
+template <typename R>
 void launch_materialize(StateSet<R> s0, StateSet<R> s1, long long n, Ctrl* ctrl, hipStream_t s);
 template <typename R>
 int launch_energy(const R* q, const SiteType* types, const DevObject* objects, int lx, int ly, int lz,
@@ -1154,6 +1157,36 @@ int qmin::force_pass(qm_lattice* L, const qm_ldg* f, bool gated) {
     FireConst fire{};
     return L->precision == QM_FP64 ? run_pass<double>(L, f, kForce, fire, gated)
                                    : run_pass<float>(L, f, kForce, fire, gated);
+}
+
+template <typename R>
+static int step_pass_t(qm_lattice* L, const qm_ldg* f, int mode, const StepArgs& step, long long* slots) {
+    // the force pass's tile and z chunk: the update adds no staging
+    choose_zchunk(L, kForce, terms_of(f));
+    MarchArgs<R> ma{};
+    BoundaryArgs<R> ba{};
+    FireConst fire{};
+    const int tile = active_tile(L, kForce);
+    fill_args<R>(L, f, ma, ba, fire, tile);
+    ma.step = step;
+    ba.step = step;
+    HIP_TRY(launch_step<R>(ma, ba, mode, terms_of(f), tile, L->stream));
+    *slots = (long long)ma.nblocks + boundary_blocks(L->nBoundary);
+    return QM_OK;
+}
+
+int qmin::step_pass(qm_lattice* L, const qm_ldg* f, int mode, const StepArgs& step, long long* slots) {
+    return L->precision == QM_FP64 ? step_pass_t<double>(L, f, mode, step, slots)
+                                   : step_pass_t<float>(L, f, mode, step, slots);
+}
+
+int qmin::share_velocity(qm_lattice* L) {
+    const int cur = L->hCtrl.cur;
+    const size_t bytes = 5 * (size_t)L->n * L->realSize;
+    char* src = static_cast<char*>(L->dSet[cur]) + bytes;
+    char* dst = static_cast<char*>(L->dSet[cur ^ 1]) + bytes;
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, L->stream));
+    return QM_OK;
 }
 
 // every change of the types ends in refresh_types, which counts them

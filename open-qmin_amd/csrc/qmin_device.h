@@ -42,7 +42,9 @@ constexpr int kReduceThreads = 256;
 constexpr int kMaxEnergyBlocks = 4096;
 constexpr int kFastWeights = 160;  // >= entries of the composed L1..L4 operator
 
-enum Mode { kForce = 0, kFire = 1 };
+// kAdamStep, kNesterovStep: the force of every site, then that minimizer's
+// update of the site, written to the other state set (one pass per iteration)
+enum Mode { kForce = 0, kFire = 1, kAdamStep = 2, kNesterovStep = 3 };
 // kMultiLinear: L1..L3, kMultiL4: L1..L4, kMultiL6: L1..L4 and L6
 enum Terms { kOneConstant = 0, kMultiLinear = 1, kMultiL4 = 2, kMultiL6 = 3 };
 
@@ -96,6 +98,17 @@ struct EnergyConst {
     const double* hvar;   // spatially varying H, planes hx, hy, hz of n (null: none)
 };
 
+// The Adam or Nesterov update that a pass in mode kAdamStep / kNesterovStep
+// applies to each site once its force is known.
+struct StepArgs {
+    void* m;             // Adam: first moment; Nesterov: the alternate sequence (5 planes of n, lattice precision)
+    void* v;             // Adam: second moment
+    const double* scal;  // device scalars: Adam beta1^t, beta2^t; Nesterov lambda, mu
+    double dt, b1, b2, eps;
+    int scheduled;       // Nesterov: momentum 1 + gamma from lambda
+    int pad;
+};
+
 template <typename R>
 struct StateSet {
     R* q;  // 5 planes of n
@@ -118,6 +131,7 @@ struct MarchArgs {
     const double* hvar; // spatially varying H, planes hx, hy, hz of n (null: none)
     double hvarP;       // deltaChi mu0
     int storeF;         // FIRE pass that keeps only Q and v: store F of the sites it finishes as well
+    StepArgs step;      // modes kAdamStep, kNesterovStep
 };
 
 template <typename R>
@@ -138,6 +152,7 @@ struct BoundaryArgs {
     double* trace;          // [traceCap][6]
     const double* hvar;
     double hvarP;
+    StepArgs step;          // modes kAdamStep, kNesterovStep
 };
 
 // Per-lattice arguments of the batched FIRE scalar step.

@@ -86,6 +86,16 @@ int prepare(qm_lattice* L, const qm_ldg* f);
 // F = forces(Q) into the current state set; gated passes are skipped on the
 // device once the control block reports done
 int force_pass(qm_lattice* L, const qm_ldg* f, bool gated);
+// One Adam or Nesterov iteration's lattice passes (mode kAdamStep or
+// kNesterovStep), skipped on the device once the control block reports done:
+// the force of every site and the minimizer's update, written with that force
+// to the other state set; the partial sums of |F|^2 go to the lattice's
+// partials buffer, three doubles per slot, and *slots receives their count.
+// The caller's finalize sums them, advances the minimizer and flips the set.
+int step_pass(qm_lattice* L, const qm_ldg* f, int mode, const StepArgs& step, long long* slots);
+// copies U of the current state set into the other one, so that passes that
+// alternate the sets without touching U keep the FIRE velocity
+int share_velocity(qm_lattice* L);
 int sync_ctrl_to_host(qm_lattice* L);
 // makes a pending FIRE velocity mix concrete in U of the current state set
 int materialize(qm_lattice* L);

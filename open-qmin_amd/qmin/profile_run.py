@@ -8,10 +8,11 @@
   python open-qmin_amd/qmin/profile_run.py copy [--mib 512] [--steps 10]
       a plain device-to-device copy of known size, used to calibrate the
       FETCH_SIZE / WRITE_SIZE counters (bytes the counter reports per byte moved).
-  python open-qmin_amd/qmin/profile_run.py minimizers --size 384 [--precision fp64]
+  python open-qmin_amd/qmin/profile_run.py minimizers --size 384 [--precision fp64] [--workload full|bench]
       the full-elasticity workload (qmin.workloads.full_elasticity: L1..L4 and
-      L6, a uniform E field, four homeotropic spheres); times per iteration of
-      FIRE, Adam, Nesterov and online L-BFGS (memory 8).
+      L6, a uniform E field, four homeotropic spheres), or the bench's walls +
+      sphere workload (L1..L3); times per iteration of FIRE, Adam, Nesterov and
+      online L-BFGS (memory 8).
   python open-qmin_amd/qmin/profile_run.py warmstart --size 128 [--steps 5]
       the warm-start network's BF16 matrix-core convolutions on an L^3 lattice
       with five spheres: --steps forward passes (inference), then --steps
@@ -51,10 +52,14 @@ def ensemble(qmin, count, steps):
     print("%d iterations each: %.3f s, %.3e site-updates/s" % (steps, t1 - t0, updates / (t1 - t0)), flush=True)
 
 
-def minimizers(qmin, L, precision, name):
+def minimizers(qmin, L, precision, name, workload="full"):
     import time
-    from qmin import workloads
-    lat, f = workloads.full_elasticity(qmin, L, precision)
+    if workload == "full":
+        from qmin import workloads
+        lat, f = workloads.full_elasticity(qmin, L, precision)
+    else:
+        from bench import build_workload
+        lat, f = build_workload(qmin, L, precision)
     print("%d^3 %s, %d boundary sites" % (L, name, lat.n_boundary()), flush=True)
     runs = (("FIRE", qmin.FIRE(dt=5e-4, dt_max=2e-2, fc=1e-12, max_iter=10 ** 9), 20),
             ("Adam", qmin.Adam(dt=1e-4, fc=1e-12, max_iter=10 ** 9), 10),
@@ -114,8 +119,9 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--mib", type=int, default=512)
     ap.add_argument("--tile", type=int, default=-1, help="tile of the FIRE lattice pass (-1: the library's default)")
-    ap.add_argument("--workload", choices=["bench", "full"], default="bench",
-                    help="fire: the bench workload, or the full-elasticity one (L1-L4, L6, E field, four spheres)")
+    ap.add_argument("--workload", choices=["bench", "full"], default=None,
+                    help="the bench workload, or the full-elasticity one (L1-L4, L6, E field, four spheres); "
+                         "default: bench for fire, full for minimizers")
     args = ap.parse_args()
     if args.what == "copy":
         import torch
@@ -142,11 +148,12 @@ def main():
         ensemble(qmin, args.count, args.steps)
         return
     if args.what == "minimizers":
-        minimizers(qmin, args.size, qmin.QM_FP64 if args.precision == "fp64" else qmin.QM_FP32, args.precision)
+        minimizers(qmin, args.size, qmin.QM_FP64 if args.precision == "fp64" else qmin.QM_FP32, args.precision,
+                   args.workload or "full")
         return
     from bench import build_workload, fire_params
     precision = qmin.QM_FP64 if args.precision == "fp64" else qmin.QM_FP32
-    if args.workload == "full":
+    if (args.workload or "bench") == "full":
         from qmin import workloads
         lat, force = workloads.full_elasticity(qmin, args.size, precision)
     else:

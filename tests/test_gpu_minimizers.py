@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import qmin
-from test_gpu_parity import ALL, MULTI, ONE, build_pair
+from test_gpu_parity import ALL, MULTI, ONE, assert_forces_close, build_pair
 
 pytestmark = pytest.mark.gpu
 
@@ -97,6 +97,60 @@ def test_nesterov_then_fire_keeps_state_consistent():
     qmin.Nesterov(dt=1e-2, mu=0.5, max_iter=10).minimize(f, g)
     qmin.FIRE(max_iter=15).minimize(f, g)
     np.testing.assert_allclose(g.q(), o.q(), rtol=0, atol=1e-11)
+
+
+@pytest.mark.parametrize("kind", ["adam", "nesterov"])
+def test_fused_step_keeps_velocity_and_stored_force(kind):
+    """An Adam or Nesterov iteration is one lattice pass that writes the new Q
+    and the force to the other state set and leaves the FIRE velocity alone.
+    FIRE, then 7 iterations (an odd number of set flips) in two calls, then FIRE
+    again follows the oracle; the velocity is unchanged by the 7 iterations; and
+    the stored force is the force at the Q before the last update, which is
+    what the reference's force array holds after its last step
+    (energyMinimizerAdam.cpp:25-54, energyMinimizerNesterovAG.cpp:75-92)."""
+    o, g, f = build_pair((16, 14, 12), MULTI, 0, True)
+    o.fire(max_iter=10)
+    qmin.FIRE(max_iter=10).minimize(f, g)
+    v_before = g.v()
+    assert np.abs(v_before).max() > 0
+    if kind == "adam":
+        o.adam(dt=1e-3, max_iter=7)
+        m = qmin.Adam(dt=1e-3, max_iter=6)
+    else:
+        o.nesterov(dt=1e-2, mu=0.5, max_iter=7)
+        m = qmin.Nesterov(dt=1e-2, mu=0.5, max_iter=6)
+    m.minimize(f, g)
+    q6 = g.q()
+    m.set_max_iterations(7)
+    it, _ = m.minimize(f, g)
+    assert it == 7
+    np.testing.assert_allclose(g.q(), o.q(), rtol=0, atol=1e-11)
+    assert_forces_close(g.f(), o.forces(q6))
+    assert np.array_equal(g.v(), v_before)
+    o.fire(max_iter=10)
+    qmin.FIRE(max_iter=10).minimize(f, g)
+    np.testing.assert_allclose(g.q(), o.q(), rtol=0, atol=1e-11)
+
+
+@pytest.mark.parametrize("kind", ["adam", "nesterov"])
+@pytest.mark.parametrize("precision", [qmin.QM_FP64, qmin.QM_FP32])
+def test_fused_step_on_wide_tiles(kind, precision):
+    """Lattices at least 128 sites wide in x and y take the 512-thread tiles
+    (32 x 16 in FP64, 64 x 8 in FP32) for the Adam and Nesterov passes."""
+    o, g, f = build_pair((128, 128, 6), MULTI, 0, True, precision=precision)
+    if kind == "adam":
+        _, tr_ref = o.adam(dt=1e-3, max_iter=12)
+        m = qmin.Adam(dt=1e-3, max_iter=12)
+    else:
+        _, tr_ref = o.nesterov(dt=1e-2, mu=0.5, scheduled=True, max_iter=12)
+        m = qmin.Nesterov(dt=1e-2, mu=0.5, scheduled=True, max_iter=12)
+    m.set_trace(16)
+    it, _ = m.minimize(f, g)
+    assert it == 12
+    if precision == qmin.QM_FP64:
+        check(m.trace(), tr_ref, g, o)
+    else:
+        np.testing.assert_allclose(m.trace(), tr_ref, rtol=1e-4)
 
 
 def test_fp32_minimizers_track_fp64():
