@@ -80,8 +80,10 @@ int qm_lattice_set_tile(qm_lattice* lat, int tile);
 int qm_lattice_autotune(qm_lattice* lat, qm_ldg* f, int* tile, int* zchunk, double* ms);
 /* converts the device state (Q, v, F) to the other storage precision, e.g. for an
  * FP64 polish after an FP32 minimization; minimizers bound to the lattice other
- * than FIRE restart their history.  No reference counterpart (the reference is
- * FP64 throughout). */
+ * than FIRE restart their history.  Converting to FP64 writes back the object
+ * sites' Q in FP64 (as last given by the host), so the fixed boundary values carry
+ * no float rounding.  No reference counterpart (the reference is FP64
+ * throughout). */
 int qm_lattice_set_precision(qm_lattice* lat, int precision);
 /* AoS host <-> device state; which = QM_FIELD_Q/V/F (simpleModel::returnPositions/
  * returnVelocities/returnForces, src/model/simpleModel.h:63-73) */

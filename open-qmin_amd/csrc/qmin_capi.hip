@@ -13,6 +13,7 @@
 #include <random>
 #include <sstream>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "qmin_internal.h"
@@ -184,6 +185,22 @@ int qmin::ensure_partials(qm_lattice* L, size_t count) {
 // AoS host field -> SoA device plane set (current set)
 static int upload_field(qm_lattice* L, int which, const double* aos) {
     const long long n = L->n;
+    if (which == QM_FIELD_Q) {
+        // The objects' FP64 Q, restored when an FP32 state converts back to
+        // FP64.  On an FP32 lattice a value equal to the kept one in FP32 is the
+        // same value come back through a download (random_init and the like
+        // rewrite only liquid-crystal sites): the FP64 value is kept.
+        const bool fp32 = L->precision == QM_FP32;
+        for (HostObject& ob : L->objects) {
+            ob.qb.resize(5 * ob.sites.size(), 0.0);
+            for (size_t k = 0; k < ob.sites.size(); ++k) {
+                const double* v = aos + 5 * ob.sites[k];
+                double* kept = &ob.qb[5 * k];
+                for (int c = 0; c < 5; ++c)
+                    if (!fp32 || (float)v[c] != (float)kept[c]) kept[c] = v[c];
+            }
+        }
+    }
     if (L->precision == QM_FP64) {
         std::vector<double> soa(5 * n);
         for (long long i = 0; i < n; ++i)
@@ -282,6 +299,64 @@ __global__ void convert_kernel(const S* src, D* dst, long long count) {
         dst[i] = D(src[i]);
 }
 
+// q[c n + sites[k]] = vals[5 k + c] (each site listed once)
+__global__ void scatter_object_q_kernel(double* q, long long n, const int64_t* sites, long long m,
+                                        const double* vals) {
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < m; k += (long long)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int c = 0; c < 5; ++c) q[c * n + sites[k]] = vals[5 * k + c];
+    }
+}
+
+// Object sites of an FP64 state converted from FP32 get back their FP64 Q (the
+// fixed boundary values the FP32 state rounded), in both state sets.  A site
+// listed by several objects takes the value of the last, as when they were
+// added.
+static int restore_object_q(qm_lattice* L) {
+    std::vector<int64_t> sites;
+    std::vector<double> vals;
+    std::unordered_map<int64_t, size_t> at;
+    for (const HostObject& ob : L->objects) {
+        if (ob.qb.size() != 5 * ob.sites.size()) continue;
+        for (size_t k = 0; k < ob.sites.size(); ++k) {
+            auto it = at.find(ob.sites[k]);
+            size_t slot;
+            if (it == at.end()) {
+                slot = sites.size();
+                at.emplace(ob.sites[k], slot);
+                sites.push_back(ob.sites[k]);
+                vals.resize(vals.size() + 5);
+            } else {
+                slot = it->second;
+            }
+            std::memcpy(&vals[5 * slot], &ob.qb[5 * k], 5 * sizeof(double));
+        }
+    }
+    if (sites.empty()) return QM_OK;
+    const long long m = (long long)sites.size();
+    int64_t* dSites = nullptr;
+    double* dVals = nullptr;
+    if (hipMalloc(&dSites, m * sizeof(int64_t)) != hipSuccess) return fail(QM_ENOMEM, "object Q: out of device memory");
+    if (hipMalloc(&dVals, 5 * m * sizeof(double)) != hipSuccess) {
+        (void)hipFree(dSites);
+        return fail(QM_ENOMEM, "object Q: out of device memory");
+    }
+    hipError_t err = hipMemcpyAsync(dSites, sites.data(), m * sizeof(int64_t), hipMemcpyHostToDevice, L->stream);
+    if (err == hipSuccess)
+        err = hipMemcpyAsync(dVals, vals.data(), 5 * m * sizeof(double), hipMemcpyHostToDevice, L->stream);
+    for (int s = 0; s < 2 && err == hipSuccess; ++s) {
+        const int blocks = (int)std::min<long long>((m + 255) / 256, 4096);
+        scatter_object_q_kernel<<<blocks, 256, 0, L->stream>>>(static_cast<double*>(L->dSet[s]), L->n, dSites, m,
+                                                               dVals);
+        err = hipGetLastError();
+    }
+    if (err == hipSuccess) err = hipStreamSynchronize(L->stream);
+    (void)hipFree(dSites);
+    (void)hipFree(dVals);
+    if (err != hipSuccess) return fail(QM_EHIP, std::string("restoring object Q: ") + hipGetErrorString(err));
+    return QM_OK;
+}
+
 // Converts the device state (Q, velocity with any pending FIRE mix applied,
 // forces) to the other storage precision in place of the old buffers.
 int qm_lattice_set_precision(qm_lattice* L, int precision) {
@@ -313,6 +388,7 @@ int qm_lattice_set_precision(qm_lattice* L, int precision) {
     }
     L->precision = precision;
     L->realSize = realSize;
+    if (precision == QM_FP64) return restore_object_q(L);
     return QM_OK;
 }
 
@@ -471,7 +547,7 @@ int qm_lattice_add_object(qm_lattice* L, const int32_t* xyz, int64_t n, const do
         sites.push_back(i);
         std::memcpy(&q[5 * i], qb + 5 * k, 5 * sizeof(double));
     }
-    L->objects.push_back({btype, W, S0});
+    L->objects.push_back({btype, W, S0, {}, {}, {}});
     const int j = (int)L->objects.size();
     for (int64_t i : sites) L->types[i] = j;
     // surface sites in order of discovery, first occurrence kept

@@ -20,6 +20,7 @@ struct HostObject {
     double W, S0;
     std::vector<int64_t> sites;     // cubicLattice::boundarySites: creation order
     std::vector<int64_t> surface;   // cubicLattice::surfaceSites: order of discovery
+    std::vector<double> qb;         // FP64 Q of each entry of sites (5 per site), as last set from the host
 };
 
 struct qm_lattice {

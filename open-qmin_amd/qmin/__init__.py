@@ -160,6 +160,16 @@ def library(path=LIB_PATH):
     return _lib
 
 
+def check_torch_runtime_order(what):
+    """torch must be imported before libqmin.so is loaded when the two share
+    device memory or streams: torch ships a HIP runtime with the same soname as
+    the system's, the first one loaded serves the process, and torch finds no
+    GPU under the system runtime.  Raises QminError if the library came first."""
+    if _lib is not None and not _torch_loaded_first:
+        raise QminError("import torch before the first qmin call when %s "
+                        "(one HIP runtime for torch and libqmin.so)" % what)
+
+
 def exported_symbols():
     return ["qm_last_error"] + list(_API)
 
@@ -201,7 +211,10 @@ class Lattice:
     def set_stream(self, stream):
         """Run the lattice's work on a caller's HIP stream (an integer handle,
         e.g. torch.cuda.Stream().cuda_stream, default priority); None returns
-        to the lattice's own stream."""
+        to the lattice's own stream.  A torch stream needs torch imported before
+        the library was loaded (check_torch_runtime_order)."""
+        if stream and "torch" in sys.modules:
+            check_torch_runtime_order("a torch stream is passed to a lattice")
         _check(self.lib.qm_lattice_set_stream(self.h, ctypes.c_void_p(stream) if stream else None))
 
     def set_hfield(self, h3):

@@ -36,13 +36,8 @@ def _torch():
 
 
 def _check_runtime_order():
-    """torch must be imported before libqmin.so is loaded: torch ships its own
-    HIP runtime, and once the library has loaded the system one first, torch
-    finds no GPU.  Importing torch first makes both use one runtime."""
-    from . import _lib, _torch_loaded_first
-    if _lib is not None and not _torch_loaded_first:
-        raise RuntimeError("import torch before the first qmin call when the warm start is used "
-                           "(one HIP runtime for torch and libqmin.so)")
+    from . import check_torch_runtime_order
+    check_torch_runtime_order("the warm start is used")
 
 
 def _ptr(t):

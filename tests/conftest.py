@@ -13,6 +13,14 @@ REFERENCE_CHECKOUT = "/root/reference"
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library on the device)")
+    markexpr = config.getoption("markexpr", "") or ""
+    if "gpu" in markexpr and "not gpu" not in markexpr:
+        # torch before libqmin.so, so that the tests that hand torch tensors or
+        # streams to the library find one HIP runtime (qmin.check_torch_runtime_order)
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
 
 
 @pytest.fixture(scope="session")

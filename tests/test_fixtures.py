@@ -25,6 +25,7 @@ import pytest
 
 import pyoracle as po
 from conftest import REPO
+from test_gpu_parity import assert_forces_close
 
 ASSETS = os.path.join(REPO, "tests", "golden", "assets")
 A, B, C = po.rescaled_phase()
@@ -298,7 +299,7 @@ def test_more_than_127_objects_match_oracle():
     f = qmin.LandauDeGennesLC(A, B, C, 2.32, 0.5, 0.3, semantics=1)
     ref = o.forces()
     got = f.compute_forces(g)
-    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    assert_forces_close(got, ref)
     e_ref, _ = o.energy()
     e_got, _ = f.compute_energy(g)
     assert abs(e_got - e_ref) <= 1e-12 * abs(e_ref)
@@ -317,6 +318,8 @@ def test_lattice_on_a_caller_stream():
     returns to the lattice's own stream."""
     torch = pytest.importorskip("torch")
     import qmin
+    if qmin._lib is not None and not qmin._torch_loaded_first:
+        pytest.skip("libqmin.so was loaded before torch in this process (the -m gpu session imports torch first)")
     res = []
     for use in (False, True):
         g = qmin.Lattice(24, 20, 16)

@@ -21,6 +21,7 @@ import pytest
 import pyoracle as po
 import qmin
 from qmin import workloads
+from test_gpu_parity import assert_forces_close
 
 pytestmark = pytest.mark.gpu
 
@@ -141,8 +142,7 @@ def test_walls_sphere_256_full_size():
         f = qmin.LandauDeGennesLC(A, B, C, 2.32, 0.5, 0.3, semantics=sem)
         ref = o.forces()
         got = f.compute_forces(g)
-        err = np.abs(got - ref).max()
-        assert err <= 1e-12 * np.abs(ref).max(), (sem, err)
+        assert_forces_close(got, ref)
         del ref, got
     it_ref, tr_ref = o.fire(max_iter=3)
     m = qmin.FIRE(max_iter=3)

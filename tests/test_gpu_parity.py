@@ -1,7 +1,7 @@
 """HIP path (libqmin.so on the MI355X) against the CPU oracle on seeded inputs.
 
-Tolerances: FP64 forces within 1e-12 of the largest oracle force component on
-the lattice (the north-star bar is 1e-12 relative); FP64 energies within 1e-12
+Tolerances: FP64 forces within 1e-12 relative at every site (assert_forces_close;
+FP32 states within 2e-5 of the lattice's largest force); FP64 energies within 1e-12
 relative; FIRE trajectories within 1e-10 relative per traced scalar (FP64
 reduction order differs from the oracle's serial sums); integer data bit-exact.
 """
@@ -45,6 +45,26 @@ def build_pair(shape, params, semantics, objects=True, efield=None, seed=1234, p
 
 
 def assert_forces_close(got, ref, rel=1e-12):
+    """FP64 forces against the oracle, per site: |F_got - F_ref| <= rel |F_ref|
+    (2-norms of the five components) wherever F_ref is not zero, and F_got == 0
+    exactly where it is (object sites).  Measured on the seeded test lattices:
+    at most 5.4e-15."""
+    got = np.asarray(got).reshape(-1, 5)
+    ref = np.asarray(ref).reshape(-1, 5)
+    nref = np.linalg.norm(ref, axis=1)
+    err = np.linalg.norm(got - ref, axis=1)
+    zero = nref == 0
+    assert np.all(err[zero] == 0), "force on %d zero-force sites" % int((err[zero] != 0).sum())
+    rel_err = err[~zero] / nref[~zero]
+    worst = int(np.argmax(rel_err)) if rel_err.size else 0
+    assert rel_err.size == 0 or rel_err[worst] <= rel, \
+        "per-site relative error %g > %g (|F_ref| there %g)" % (rel_err[worst], rel, nref[~zero][worst])
+
+
+def assert_fp32_forces_close(got, ref, rel=2e-5):
+    """FP32 state against the FP64 oracle: the state itself is rounded to float,
+    so the bound is relative to the largest force of the lattice, not per site
+    (where a small force is the difference of terms whose rounding is larger)."""
     scale = max(np.abs(ref).max(), 1e-300)
     err = np.abs(got - ref).max()
     assert err <= rel * scale, "max |dF| = %g, scale %g" % (err, scale)
@@ -85,7 +105,7 @@ def test_forces_fp32_close():
     o, g, f = build_pair((40, 20, 16), MULTI, 1, True, precision=qmin.QM_FP32)
     ref = o.forces()
     got = f.compute_forces(g)
-    assert_forces_close(got, ref, rel=2e-5)
+    assert_fp32_forces_close(got, ref)
 
 
 @pytest.mark.parametrize("params", [ONE, MULTI, ALL])
@@ -172,37 +192,40 @@ def test_fire_stops_on_tolerance_like_reference():
 
 @pytest.mark.parametrize("mode", ["fp64", "mixed"])
 def test_converged_state_matches_oracle(mode):
-    """At a fixed tolerance (forceMax < 1e-9, walls + sphere, L1-L3) the converged
+    """At a fixed tolerance (forceMax < 1e-11, walls + sphere, L1-L3) the converged
     energy and the largest-eigenvalue defect field against the oracle's FIRE run.
 
-    FP64 follows the oracle's trajectory and stops at the same iteration: energy
-    and defect field within 1e-9 relative (measured: 2e-15 and 8e-16).  Mixed
-    (FP32 to 1e-7, then the FP64 polish to 1e-9; on 4320 sites FP32 rounding alone
-    leaves forceMax = |F|/N near 1e-8, so the switch comes earlier than at 256^3)
-    ends in the same minimum but at another point within the tolerance, 16
-    iterations later: measured energy error 6.1e-9 relative, defect field error
-    1.2e-6 absolute (2.2e-6 of its largest value); bounds 2e-8 and 1e-5."""
+    FP64 follows the oracle's trajectory and stops at the same iteration (2856):
+    energy and defect field within 1e-9 relative (measured: 1.6e-15 and 1.3e-15).
+    Mixed runs FP32 to forceMax < 1e-7 (on 4320 sites FP32 rounding alone leaves
+    forceMax = |F|/N near 1e-8, so the switch comes earlier than at 256^3), then
+    converts, which gives the object sites back their FP64 boundary values, and
+    polishes in FP64 to 1e-11.  It ends in the same minimum at another point
+    within the tolerance: energy within 1e-9 relative (measured 6.3e-11; 7.0e-9
+    while the converted state kept FP32-rounded boundary values), defect field
+    1.2e-8 of its largest value, bound 1e-7."""
+    fc = 1e-11
     shape = (20, 18, 12)
     prec = qmin.QM_FP64 if mode == "fp64" else qmin.QM_FP32
     o, g, f = build_pair(shape, MULTI, 1, True, precision=prec)
-    it_ref, tr_ref = o.fire(fc=1e-9, max_iter=100000)
-    assert tr_ref[-1, 0] < 1e-9
-    m = qmin.FIRE(fc=1e-9 if mode == "fp64" else 1e-7, max_iter=100000)
+    it_ref, tr_ref = o.fire(fc=fc, max_iter=200000)
+    assert tr_ref[-1, 0] < fc
+    m = qmin.FIRE(fc=fc if mode == "fp64" else 1e-7, max_iter=200000)
     st = m.minimize(f, g)
     if mode == "mixed":
         assert st.converged
         g.set_precision(qmin.QM_FP64)
-        m.set_force_cutoff(1e-9)
+        m.set_force_cutoff(fc)
         st = m.minimize(f, g)
-    assert st.converged and st.forceMax < 1e-9
+    assert st.converged and st.forceMax < fc
     if mode == "fp64":
         assert st.iterations == it_ref
     e_ref, _ = o.energy()
     e_got, _ = f.compute_energy(g)
     lc = o.types() <= 0
     d_ref, d_got = o.defects(0)[lc], g.defects(0)[lc]
-    e_tol, d_tol = (1e-9, 1e-9) if mode == "fp64" else (2e-8, 1e-5)
-    assert abs(e_got - e_ref) <= e_tol * abs(e_ref)
+    d_tol = 1e-9 if mode == "fp64" else 1e-7
+    assert abs(e_got - e_ref) <= 1e-9 * abs(e_ref)
     assert np.abs(d_got - d_ref).max() <= d_tol * np.abs(d_ref).max()
 
 
@@ -293,7 +316,9 @@ def test_fire_continues_across_calls():
 
 
 def test_precision_conversion_round_trip():
-    """FP64 -> FP32 rounds Q, v, F to float; FP32 -> FP64 is exact; FIRE continues."""
+    """FP64 -> FP32 rounds Q, v, F to float; FP32 -> FP64 keeps the float values
+    of the liquid crystal and gives the object sites back their FP64 Q (the
+    fixed boundary values); FIRE continues."""
     o, g, f = build_pair((14, 12, 10), MULTI, 1, True)
     m = qmin.FIRE(max_iter=5)
     m.minimize(f, g)
@@ -303,10 +328,28 @@ def test_precision_conversion_round_trip():
     assert np.array_equal(g.v(), v64.astype(np.float32).astype(np.float64))
     assert np.array_equal(g.f(), f64.astype(np.float32).astype(np.float64))
     g.set_precision(qmin.QM_FP64)
-    assert np.array_equal(g.q(), q64.astype(np.float32).astype(np.float64))
+    obj = g.types() > 0
+    q = g.q()
+    assert obj.any() and not np.array_equal(q64[obj], q64[obj].astype(np.float32).astype(np.float64))
+    assert np.array_equal(q[obj], q64[obj])
+    assert np.array_equal(q[~obj], q64[~obj].astype(np.float32).astype(np.float64))
     m.set_max_iterations(10)
     st = m.minimize(f, g)
     assert st.iterations == 10 and np.isfinite(st.forceMax)
+
+
+def test_fp32_lattice_converted_to_fp64_has_exact_object_q():
+    """Objects added to an FP32 lattice, a seeded start (which downloads Q and
+    uploads it again), FIRE in FP32, then the conversion: the object sites hold
+    the FP64 boundary values bit for bit, as on a lattice built in FP64, so the
+    FP64 polish minimizes the same energy as an FP64 run."""
+    o, g, f = build_pair((18, 14, 12), MULTI, 1, True)
+    _, g32, f32 = build_pair((18, 14, 12), MULTI, 1, True, precision=qmin.QM_FP32)
+    qmin.FIRE(max_iter=20).minimize(f32, g32)
+    g32.set_precision(qmin.QM_FP64)
+    obj = g.types() > 0
+    assert np.array_equal(g32.types(), g.types())
+    assert np.array_equal(g32.q()[obj], o.q()[obj])
 
 
 @pytest.mark.parametrize("params", [MULTI, ALL], ids=["L1-L3", "L1-L6"])
@@ -370,8 +413,10 @@ def test_spatially_varying_field_forces_energy(semantics, params, precision):
     o.set_hfield_varying(h, 1.0, 1.0, 0.7)
     g.set_hfield(h)
     f.set_varying_hfield(1.0, 1.0, 0.7)
-    rel = 1e-12 if precision == qmin.QM_FP64 else 2e-5
-    assert_forces_close(f.compute_forces(g), o.forces(), rel=rel)
+    if precision == qmin.QM_FP64:
+        assert_forces_close(f.compute_forces(g), o.forces())
+    else:
+        assert_fp32_forces_close(f.compute_forces(g), o.forces())
     e_ref, c_ref = o.energy()
     e_got, c_got = f.compute_energy(g)
     np.testing.assert_allclose(e_got, e_ref, rtol=1e-12 if precision == qmin.QM_FP64 else 1e-5)
