@@ -452,16 +452,19 @@ __device__ __forceinline__ int xcd_logical(int b, int nb) {
 }
 
 // Waves per SIMD asked of the compiler for the 256-thread tiles: four (at most
-// 128 VGPRs) for the FP64 force-only pass with L6, which needs 132 otherwise and
+// 128 VGPRs) for the FP64 force-only pass with L6 on 32 x 8 (its tile by
+// default), which needs 132 otherwise and
 // spills 4 at 128, and is 2% faster so at 384^3 (2.68-2.69 against 2.74-2.77 ms);
 // the Adam and Nesterov passes with L6 spill 18 and 11 registers at 128 and are
 // slower (5.31 against 4.69 ms, 4.00 against 3.55 ms), so they and the rest keep
 // what their registers give.
-template <typename R, int MODE, int TERMS>
-constexpr int march_min_waves() { return (sizeof(R) == 8 && TERMS == kMultiL6 && MODE == kForce) ? 4 : 1; }
+template <typename R, int MODE, int TERMS, int TILE>
+constexpr int march_min_waves() {
+    return (sizeof(R) == 8 && TERMS == kMultiL6 && MODE == kForce && TILE == kTile32x8) ? 4 : 1;
+}
 
 template <typename R, int MODE, int TERMS, int TILE>
-__global__ void __launch_bounds__(kTileThreads[TILE], (march_min_waves<R, MODE, TERMS>())) march_kernel(MarchArgs<R> A) {
+__global__ void __launch_bounds__(kTileThreads[TILE], (march_min_waves<R, MODE, TERMS, TILE>())) march_kernel(MarchArgs<R> A) {
     march_body<R, MODE, TERMS, TILE>(A, xcd_logical<false>(blockIdx.x, A.nblocks), blockIdx.x);
 }
 

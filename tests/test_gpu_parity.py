@@ -443,6 +443,30 @@ def test_spatially_varying_field_fire_and_file(tmp_path):
     np.testing.assert_allclose(g.q(), o.q(), rtol=0, atol=1e-11)
 
 
+@pytest.mark.parametrize("kind", ["adam", "nesterov"])
+def test_spatially_varying_field_adam_nesterov(kind):
+    """The Adam and Nesterov passes (force and update in one pass) with a
+    spatially varying H field and a uniform E field, L1-L6, objects: traces and
+    final Q against the oracle."""
+    shape = (22, 16, 12)
+    o, g, f = build_pair(shape, ALL, 0, True, efield=(0.1, 0.0, 0.4))
+    h = _random_field(g.n, 13)
+    o.set_hfield_varying(h, 1.0, 1.0, 0.6)
+    g.set_hfield(h)
+    f.set_varying_hfield(1.0, 1.0, 0.6)
+    if kind == "adam":
+        _, tr_ref = o.adam(dt=1e-3, max_iter=25)
+        m = qmin.Adam(dt=1e-3, max_iter=25)
+    else:
+        _, tr_ref = o.nesterov(dt=1e-2, mu=0.5, scheduled=True, max_iter=25)
+        m = qmin.Nesterov(dt=1e-2, mu=0.5, scheduled=True, max_iter=25)
+    m.set_trace(32)
+    it, _ = m.minimize(f, g)
+    assert it == 25
+    np.testing.assert_allclose(m.trace(), tr_ref, rtol=1e-10)
+    np.testing.assert_allclose(g.q(), o.q(), rtol=0, atol=1e-11)
+
+
 def test_varying_field_requires_lattice_field():
     _, g, f = build_pair((12, 10, 9), ONE, 1, False)
     f.set_varying_hfield(1.0, 1.0, 0.5)
