@@ -17,11 +17,9 @@
 // the reference's expressions read from its source, and closed forms and
 // invariances (plane wave, lattice rotations, locality, first-derivative rules
 // on a quadratic field) for the rest.
-// Parity unpinned: orc_object_force's area vector at sites with objects on
-// both sides of one axis (checked with the force on spheres by symmetry and
-// translation properties only; the reference's force class that computes it
-// needs MPI and is not compiled here).  Flat walls and corners are pinned in
-// closed form.
+// orc_object_force (the reference's force class that computes it needs MPI and
+// is not compiled here) is pinned in closed form on flat walls, at corners and
+// between two walls one site apart (objects on both sides of one axis).
 #pragma once
 #include <stdint.h>
 

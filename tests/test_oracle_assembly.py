@@ -455,3 +455,56 @@ def test_object_force_at_a_corner_closed_form(types_xz):
     want = [ly * (e[lx - 1] - e[1]), 0.0, -ly * sum(e.values())]
     assert abs(want[0]) > 1e-3 * abs(want[2])
     np.testing.assert_allclose(o.object_force(0), want, rtol=1e-12, atol=1e-13)
+
+
+@pytest.mark.parametrize("types_12", [(0, 1), (1, 0)])
+def test_object_force_between_two_walls_closed_form(types_12):
+    """computeObjectForces at sites with objects on both sides of one axis, in
+    closed form.  Walls at z = 0 (object 1) and z = 2 (object 2) on a periodic
+    lattice of Lz = 4: every liquid-crystal site (z = 1 and z = 3) has an object
+    below and above.  The reference then sets the area vector's z component
+    last from the upper neighbour, a_z = +1 (landauDeGennesLCOtherForces.cpp:
+    175-178), and takes the one-sided z derivative against the lower neighbour,
+    G = Q - Q(below), the upper object taking precedence (:124-130), though the
+    lower neighbour is an object too.  With a uniform Q0 in the liquid crystal
+    the stress is diagonal with sigma_xx = sigma_yy = e and sigma_zz = e - 2 L1
+    (G0^2 + 2 G1^2 + 2 G2^2 + G3^2 + 2 G4^2 + G0 G3) (:232-241), e the site's
+    energy density: phase + L1/2 Tr G^2 + both anchoring terms.  Both walls have
+    the same surface sites, so each takes Lx Ly (sigma_zz(1) + sigma_zz(3)) along
+    +z.  Choosing the lower neighbour's area vector or derivative instead changes
+    the result."""
+    lib = po.ref_lib()
+    lx, ly, lz = 5, 4, 4
+    L1 = 2.32
+    bt1, bt2 = types_12
+    o = po.OracleSystem(lx, ly, lz)
+    o.random_init(3, S0)
+    o.wall(2, 0, bt1, 5.3, S0)
+    o.wall(2, 2, bt2, 2.7, S0)
+    o.set_params(A, B, C, L1, semantics=1)
+    t = o.types().reshape(lz, ly, lx)
+    assert np.all(t[0] == 1) and np.all(t[2] == 2) and np.all(t[1] == -1) and np.all(t[3] == -1)
+    q0 = np.zeros(5)
+    lib.qmref_q_from_director(po.dptr(np.array([0.48, 0.6, 0.64])), S0, po.dptr(q0))
+    Q = o.q().reshape(lz, ly, lx, 5)
+    qb = {0: np.ascontiguousarray(Q[0, 0, 0]), 2: np.ascontiguousarray(Q[2, 0, 0])}
+    assert not np.allclose(qb[0], qb[2])
+    Q[1] = q0
+    Q[3] = q0
+    o.set_q(Q.reshape(-1, 5))
+
+    def sigma_zz(below):
+        G = q0 - qb[below]
+        tr_g2 = 2 * (G[0] ** 2 + G[3] ** 2 + G[0] * G[3] + G[1] ** 2 + G[2] ** 2 + G[4] ** 2)
+        d, tr = np.zeros(15), np.zeros(3)
+        q = np.ascontiguousarray(q0)
+        lib.qmref_phase(po.dptr(q), po.dptr(d), po.dptr(tr))
+        e = 0.5 * A * tr[0] + B / 3.0 * tr[1] + 0.25 * C * tr[2] + 0.5 * L1 * tr_g2
+        e += lib.qmref_anchoring_energy(bt1, 5.3, S0, po.dptr(q), po.dptr(qb[0]))
+        e += lib.qmref_anchoring_energy(bt2, 2.7, S0, po.dptr(q), po.dptr(qb[2]))
+        return e - 2 * L1 * (G[0] ** 2 + 2 * G[1] ** 2 + 2 * G[2] ** 2 + G[3] ** 2 + 2 * G[4] ** 2 + G[0] * G[3])
+
+    want = [0.0, 0.0, lx * ly * (sigma_zz(0) + sigma_zz(2))]
+    assert abs(want[2]) > 1e-6
+    np.testing.assert_allclose(o.object_force(0), want, rtol=1e-12, atol=1e-13)
+    np.testing.assert_allclose(o.object_force(1), want, rtol=1e-12, atol=1e-13)
