@@ -294,9 +294,18 @@ int qm_ensemble_fire_minimize(qm_ensemble* e, const qm_fire* params, qm_ldg* f, 
                               int* iterations, double* forceMax);
 /* tile autotuner of the batched pass, the role of kernelTuner
  * (src/utilities/kernelTuner.cpp:5-125) for ensembles: times the batched fused
- * FIRE pass with HIP events for every tile shape and z chunks {8, 16, 32, 64} on
- * the lattices' current states (which it leaves unchanged) and keeps the
- * fastest for later qm_ensemble_fire_minimize calls (outputs may be NULL). */
+ * FIRE pass with HIP events for every tile shape, and for QM_TILE_PER_LATTICE
+ * (each lattice the 8-row tile, 32 or 64 sites wide, that pads its x edge less;
+ * one launch per shape), with z chunks {8, 16, 32, 64} on the lattices' current
+ * states (which it leaves unchanged) and keeps the fastest for later
+ * qm_ensemble_fire_minimize calls (outputs may be NULL). */
+#define QM_TILE_PER_LATTICE (-2)
+/* the batched pass's tile for later qm_ensemble_fire_minimize calls: -1 the first
+ * lattice's default, QM_TILE_PER_LATTICE, or a fused-pass tile 0..3 (as
+ * qm_lattice_set_tile); zchunk 0 (whole columns up to 32 planes) or at least 4.
+ * The setting the autotuner leaves, made by hand.  A tuning knob of this
+ * implementation. */
+int qm_ensemble_set_tile(qm_ensemble* e, int tile, int zchunk);
 int qm_ensemble_autotune(qm_ensemble* e, qm_ldg* f, int* tile, int* zchunk, double* ms);
 
 /* ---- learned warm start -----------------------------------------------------

@@ -25,6 +25,8 @@ hipError_t launch_fire_finalize_batch(const FinalizeDesc* descs, EnsembleStatus*
 
 using namespace qmin;
 
+constexpr int kTilePerLattice = QM_TILE_PER_LATTICE;
+
 struct qm_ensemble {
     std::vector<qm_lattice*> lats;
     int precision = QM_FP64;
@@ -35,10 +37,14 @@ struct qm_ensemble {
     int2* dMarchMap = nullptr;
     int2* dBoundaryMap = nullptr;
     EnsembleStatus* dStatus = nullptr;
-    int marchBlocks = 0, boundaryBlocks = 0;
-    int tile = -1;     // tile shape of the batched pass (-1: the first lattice's default)
+    int boundaryBlocks = 0;
+    // work-groups of the batched fused pass per tile shape, in one map: entries
+    // [segOff[t], segOff[t] + segCount[t]) run with tile t (one launch each)
+    int segOff[kNumMarchTiles] = {0, 0, 0, 0}, segCount[kNumMarchTiles] = {0, 0, 0, 0};
+    // tile shape of the batched pass: -1 the first lattice's default, kTilePerLattice
+    // each lattice's own (below), else that tile for all
+    int tile = -1;
     int zchunk = 0;    // z chunk (0: each lattice's own setting, else whole columns up to 32 planes)
-    int usedTile = 0;  // the shape the current descriptors were built for
     hipGraphExec_t graph = nullptr;
     int graphIters = 0;
 
@@ -69,18 +75,29 @@ static std::vector<int2> xcd_order(const std::vector<int2>& logical) {
     return map;
 }
 
+// Tile of lattice L in the batched pass.  Per lattice: the 8-row tile, 32 or
+// 64 sites wide, that pads the lattice's x edge less (64 on a tie), so that a
+// 24- or 32-wide lattice does not run half-empty 64-wide tiles.
+static int ensemble_tile(const qm_ensemble* E, const qm_lattice* L) {
+    if (E->tile >= 0) return E->tile;
+    if (E->tile == kTilePerLattice) {
+        const int w32 = (L->lx + 31) / 32 * 32, w64 = (L->lx + 63) / 64 * 64;
+        return w32 < w64 ? kTile32x8 : kTile64x8;
+    }
+    // batched passes have the tiles of the fused pass only
+    return active_tile(E->lats[0], kForce);
+}
+
 template <typename R>
 static int build_descs(qm_ensemble* E, const qm_ldg* f, const qm_fire* params) {
     const int count = (int)E->lats.size();
-    // batched passes have the tiles of the fused pass only
-    const int tile = E->tile >= 0 ? E->tile : active_tile(E->lats[0], kForce);
-    E->usedTile = tile;
     std::vector<MarchArgs<R>> md(count);
     std::vector<BoundaryArgs<R>> bd(count);
     std::vector<FinalizeDesc> fd(count);
-    std::vector<int2> mlog, blog;
+    std::vector<int2> mlog[kNumMarchTiles], blog;
     for (int k = 0; k < count; ++k) {
         qm_lattice* L = E->lats[k];
+        const int tile = ensemble_tile(E, L);
         // the ensemble's tile and z chunk live in its descriptors only: the
         // lattice's own settings are restored once its arguments are filled
         const int ownTile = L->tileOverride, ownZ = L->zchunk;
@@ -94,16 +111,21 @@ static int build_descs(qm_ensemble* E, const qm_ldg* f, const qm_fire* params) {
         md[k].gate = 0;
         bd[k].gate = 0;
         const int bb = boundary_blocks(L->nBoundary);
-        for (int j = 0; j < md[k].nblocks; ++j) mlog.push_back(make_int2(k, j));
+        for (int j = 0; j < md[k].nblocks; ++j) mlog[tile].push_back(make_int2(k, j));
         for (int j = 0; j < bb; ++j) blog.push_back(make_int2(k, j));
         fd[k].ctrl = L->dCtrl;
         fd[k].fire = fc;
         fd[k].partials = L->dPartials;
         fd[k].total = (long long)md[k].nblocks + bb;
     }
-    const std::vector<int2> mmap = xcd_order(mlog);
+    std::vector<int2> mmap;
+    for (int t = 0; t < kNumMarchTiles; ++t) {
+        const std::vector<int2> seg = xcd_order(mlog[t]);
+        E->segOff[t] = (int)mmap.size();
+        E->segCount[t] = (int)seg.size();
+        mmap.insert(mmap.end(), seg.begin(), seg.end());
+    }
     E->release_device();
-    E->marchBlocks = (int)mmap.size();
     E->boundaryBlocks = (int)blog.size();
     HIP_TRY(hipMalloc(&E->dMarchDescs, count * sizeof(MarchArgs<R>)));
     HIP_TRY(hipMalloc(&E->dBoundaryDescs, count * sizeof(BoundaryArgs<R>)));
@@ -119,10 +141,21 @@ static int build_descs(qm_ensemble* E, const qm_ldg* f, const qm_fire* params) {
     return QM_OK;
 }
 
+// the batched fused pass: one launch per tile shape in use
 template <typename R>
-static int enqueue_pass(qm_ensemble* E, int mode, int terms, int tile) {
-    HIP_TRY(launch_march_batch<R>(static_cast<const MarchArgs<R>*>(E->dMarchDescs), E->dMarchMap, E->marchBlocks, mode,
-                                  terms, tile, E->stream));
+static hipError_t launch_march_segments(qm_ensemble* E, int mode, int terms) {
+    for (int t = 0; t < kNumMarchTiles; ++t) {
+        if (E->segCount[t] == 0) continue;
+        hipError_t err = launch_march_batch<R>(static_cast<const MarchArgs<R>*>(E->dMarchDescs),
+                                               E->dMarchMap + E->segOff[t], E->segCount[t], mode, terms, t, E->stream);
+        if (err != hipSuccess) return err;
+    }
+    return hipSuccess;
+}
+
+template <typename R>
+static int enqueue_pass(qm_ensemble* E, int mode, int terms) {
+    HIP_TRY(launch_march_segments<R>(E, mode, terms));
     HIP_TRY(launch_boundary_batch<R>(static_cast<const BoundaryArgs<R>*>(E->dBoundaryDescs), E->dBoundaryMap,
                                      E->boundaryBlocks, mode, terms, E->stream));
     if (mode == kFire)
@@ -152,14 +185,14 @@ static int ensemble_fire(qm_ensemble* E, const qm_fire* params, qm_ldg* f, int g
         if (int rc = push_ctrl(L)) return rc;
     }
     if (int rc = build_descs<R>(E, f, params)) return rc;
-    const int terms = terms_of(f), tile = E->usedTile;
+    const int terms = terms_of(f);
     // sim->computeForces() at the start of minimize, on every lattice at once
-    if (int rc = enqueue_pass<R>(E, kForce, terms, tile)) return rc;
+    if (int rc = enqueue_pass<R>(E, kForce, terms)) return rc;
     // capture graphIters FIRE iterations once, replay until every lattice is done
     hipGraph_t g = nullptr;
     HIP_TRY(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < graphIters; ++k) {
-        if (int rc = enqueue_pass<R>(E, kFire, terms, tile)) {
+        if (int rc = enqueue_pass<R>(E, kFire, terms)) {
             (void)hipStreamEndCapture(E->stream, &g);
             if (g) (void)hipGraphDestroy(g);
             return rc;
@@ -222,16 +255,15 @@ static int ensemble_autotune(qm_ensemble* E, qm_ldg* f, int* tileOut, int* zchun
     double best = 1e300;
     int bestTile = 0, bestZ = 0, rc = QM_OK;
     const int reps = 3;
-    for (int tile = 0; tile < kNumMarchTiles && rc == QM_OK; ++tile) {
+    for (int tile = kTilePerLattice; tile < kNumMarchTiles && rc == QM_OK; ++tile) {
+        if (tile == -1) continue;
         for (int zc : {8, 16, 32, 64}) {
             E->tile = tile;
             E->zchunk = zc;
             if ((rc = build_descs<R>(E, f, &defaults))) break;
-            const auto* md = static_cast<const MarchArgs<R>*>(E->dMarchDescs);
-            hipError_t err = launch_march_batch<R>(md, E->dMarchMap, E->marchBlocks, kFire, terms, tile, E->stream);
+            hipError_t err = launch_march_segments<R>(E, kFire, terms);
             if (err == hipSuccess) err = hipEventRecord(e0, E->stream);
-            for (int r = 0; r < reps && err == hipSuccess; ++r)
-                err = launch_march_batch<R>(md, E->dMarchMap, E->marchBlocks, kFire, terms, tile, E->stream);
+            for (int r = 0; r < reps && err == hipSuccess; ++r) err = launch_march_segments<R>(E, kFire, terms);
             if (err == hipSuccess) err = hipEventRecord(e1, E->stream);
             if (err == hipSuccess) err = hipEventSynchronize(e1);
             float t = 0;
@@ -297,6 +329,16 @@ int qm_ensemble_autotune(qm_ensemble* E, qm_ldg* f, int* tile, int* zchunk, doub
         if (L->precision != E->precision) return fail(QM_ESTATE, "a lattice changed precision after joining");
     return E->precision == QM_FP64 ? ensemble_autotune<double>(E, f, tile, zchunk, ms)
                                    : ensemble_autotune<float>(E, f, tile, zchunk, ms);
+}
+
+int qm_ensemble_set_tile(qm_ensemble* E, int tile, int zchunk) {
+    if (!E) return fail(QM_EINVAL, "null ensemble");
+    if (!(tile == -1 || tile == kTilePerLattice || (tile >= 0 && tile < kNumMarchTiles)))
+        return fail(QM_EINVAL, "ensemble tile must be -1, QM_TILE_PER_LATTICE or 0..3");
+    if (zchunk < 0 || (zchunk > 0 && zchunk < 4)) return fail(QM_EINVAL, "z chunk must be 0 (automatic) or at least 4");
+    E->tile = tile;
+    E->zchunk = zchunk;
+    return QM_OK;
 }
 
 int qm_ensemble_fire_minimize(qm_ensemble* E, const qm_fire* params, qm_ldg* f, int graphIterations, int* iterations,

@@ -15,6 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(HERE), "lib", "libqmin.so")
 
 QM_FP64, QM_FP32 = 0, 1
+TILE_PER_LATTICE = -2   # QM_TILE_PER_LATTICE: an ensemble tile choice made per lattice
 HOMEOTROPIC, DEGENERATE_PLANAR = 0, 1
 SEMANTICS_REF_CPU, SEMANTICS_REF_GPU = 0, 1
 FIELD_Q, FIELD_V, FIELD_F = 0, 1, 2
@@ -63,6 +64,7 @@ _API = {
     "qm_ensemble_destroy": [_VP],
     "qm_ensemble_fire_minimize": [_VP, _VP, _VP, _i, ctypes.POINTER(ctypes.c_int), _D],
     "qm_ensemble_autotune": [_VP, _VP, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), _D],
+    "qm_ensemble_set_tile": [_VP, _i, _i],
     "qm_lattice_autotune": [_VP, _VP, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), _D],
     "qm_lattice_set_precision": [_VP, _i],
     "qm_lattice_set_field": [_VP, _i, _D],
@@ -599,6 +601,11 @@ class Ensemble:
         _check(self.lib.qm_ensemble_autotune(self.h, force.h, ctypes.byref(tile), ctypes.byref(zc),
                                              ctypes.byref(ms)))
         return tile.value, zc.value, ms.value
+
+    def set_tile(self, tile, zchunk=0):
+        """The batched pass's tile (-1 default, TILE_PER_LATTICE, or 0..3) and z
+        chunk (0: automatic) for later fire_minimize calls."""
+        _check(self.lib.qm_ensemble_set_tile(self.h, tile, zchunk))
 
 
 def timing_enable(on=True):

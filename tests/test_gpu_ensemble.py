@@ -87,7 +87,7 @@ def test_ensemble_autotune_leaves_states_and_results_unchanged():
     q0 = [l.q() for l in lats]
     ens = qmin.Ensemble(lats)
     tile, zchunk, ms = ens.autotune(force)
-    assert tile in (0, 1, 2, 3) and zchunk in (8, 16, 32, 64) and ms > 0
+    assert tile in (qmin.TILE_PER_LATTICE, 0, 1, 2, 3) and zchunk in (8, 16, 32, 64) and ms > 0
     for l, q in zip(lats, q0):
         # equal_nan: the object site at an integer sphere centre has no direction
         # (0/0 in the reference's Q^B as well); it never enters a force
@@ -100,3 +100,29 @@ def test_ensemble_autotune_leaves_states_and_results_unchanged():
         assert it == it_ref, spec
         assert fm == pytest.approx(tr[-1, 0], rel=1e-9), spec
         np.testing.assert_allclose(lat.q(), o.q(), rtol=0, atol=1e-10, err_msg=str(spec))
+
+
+def test_ensemble_with_a_tile_per_lattice_matches_oracle():
+    """QM_TILE_PER_LATTICE: lattices 17, 24 and 32 sites wide run 32 x 8 tiles,
+    48 and 64 wide 64 x 8 (two launches of the batched pass per iteration); each
+    lattice still stops where its own oracle run does."""
+    lats = [make(s, True) for s in SPECS]
+    force = qmin.LandauDeGennesLC(A, B, C, 2.32, 0.5, 0.3, semantics=1)
+    ens = qmin.Ensemble(lats)
+    ens.set_tile(qmin.TILE_PER_LATTICE, 16)
+    its, fms = ens.fire_minimize(qmin.FIRE(fc=1e-6, max_iter=150), force, 16)
+    for spec, lat, it, fm in zip(SPECS, lats, its, fms):
+        o = make(spec, False)
+        o.set_params(A, B, C, 2.32, 0.5, 0.3, semantics=1)
+        it_ref, tr = o.fire(fc=1e-6, max_iter=150)
+        assert it == it_ref, spec
+        assert fm == pytest.approx(tr[-1, 0], rel=1e-9), spec
+        np.testing.assert_allclose(lat.q(), o.q(), rtol=0, atol=1e-10, err_msg=str(spec))
+
+
+def test_ensemble_set_tile_rejects_bad_values():
+    lats = [make(s, True) for s in SPECS[:2]]
+    ens = qmin.Ensemble(lats)
+    for tile, zc in ((4, 0), (-3, 0), (0, 2), (0, -1)):
+        with pytest.raises(qmin.QminError):
+            ens.set_tile(tile, zc)
