@@ -153,6 +153,24 @@ def test_fused_step_on_wide_tiles(kind, precision):
         np.testing.assert_allclose(m.trace(), tr_ref, rtol=1e-4)
 
 
+@pytest.mark.parametrize("kind", ["adam", "nesterov"])
+def test_fused_step_without_objects(kind):
+    """A periodic lattice with no objects: the boundary pass has no sites, and
+    the fused pass alone finishes every site."""
+    o, g, f = build_pair((18, 14, 10), ALL, 1, False)
+    assert g.n_boundary() == 0
+    if kind == "adam":
+        _, tr_ref = o.adam(dt=1e-3, max_iter=20)
+        m = qmin.Adam(dt=1e-3, max_iter=20)
+    else:
+        _, tr_ref = o.nesterov(dt=1e-2, mu=0.5, max_iter=20)
+        m = qmin.Nesterov(dt=1e-2, mu=0.5, max_iter=20)
+    m.set_trace(32)
+    it, _ = m.minimize(f, g)
+    assert it == 20
+    check(m.trace(), tr_ref, g, o)
+
+
 def test_fp32_minimizers_track_fp64():
     for name in ("adam", "nesterov", "lolbfgs"):
         o, g, f = build_pair((20, 16, 12), MULTI, 1, True, precision=qmin.QM_FP32)
