@@ -125,6 +125,8 @@ static int build_descs(qm_ensemble* E, const qm_ldg* f, const qm_fire* params) {
         E->segCount[t] = (int)seg.size();
         mmap.insert(mmap.end(), seg.begin(), seg.end());
     }
+    // the old descriptors may still be read by work queued on the ensemble's stream
+    HIP_TRY(hipStreamSynchronize(E->stream));
     E->release_device();
     E->boundaryBlocks = (int)blog.size();
     HIP_TRY(hipMalloc(&E->dMarchDescs, count * sizeof(MarchArgs<R>)));
@@ -133,11 +135,16 @@ static int build_descs(qm_ensemble* E, const qm_ldg* f, const qm_fire* params) {
     HIP_TRY(hipMalloc(&E->dMarchMap, mmap.size() * sizeof(int2)));
     HIP_TRY(hipMalloc(&E->dBoundaryMap, blog.size() * sizeof(int2)));
     HIP_TRY(hipMalloc(&E->dStatus, count * sizeof(EnsembleStatus)));
-    HIP_TRY(hipMemcpy(E->dMarchDescs, md.data(), count * sizeof(MarchArgs<R>), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(E->dBoundaryDescs, bd.data(), count * sizeof(BoundaryArgs<R>), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(E->dFinal, fd.data(), count * sizeof(FinalizeDesc), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(E->dMarchMap, mmap.data(), mmap.size() * sizeof(int2), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(E->dBoundaryMap, blog.data(), blog.size() * sizeof(int2), hipMemcpyHostToDevice));
+    // ordered on the ensemble's stream, before the launches that read them
+    HIP_TRY(hipMemcpyAsync(E->dMarchDescs, md.data(), count * sizeof(MarchArgs<R>), hipMemcpyHostToDevice, E->stream));
+    HIP_TRY(hipMemcpyAsync(E->dBoundaryDescs, bd.data(), count * sizeof(BoundaryArgs<R>), hipMemcpyHostToDevice,
+                           E->stream));
+    HIP_TRY(hipMemcpyAsync(E->dFinal, fd.data(), count * sizeof(FinalizeDesc), hipMemcpyHostToDevice, E->stream));
+    HIP_TRY(hipMemcpyAsync(E->dMarchMap, mmap.data(), mmap.size() * sizeof(int2), hipMemcpyHostToDevice, E->stream));
+    HIP_TRY(hipMemcpyAsync(E->dBoundaryMap, blog.data(), blog.size() * sizeof(int2), hipMemcpyHostToDevice,
+                           E->stream));
+    // the host vectors end with this function
+    HIP_TRY(hipStreamSynchronize(E->stream));
     return QM_OK;
 }
 
