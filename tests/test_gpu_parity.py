@@ -94,6 +94,67 @@ def test_forces_fp64(semantics, params, objects, shape):
     assert_forces_close(got, ref)
 
 
+def _boundary_cases(types, shape):
+    """getBoundaryCase of every type -1 site (lcForces.h; xx, yy, zz = 1 for an
+    object above, 2 for one below, below winning), from the site types."""
+    lx, ly, lz = shape
+    obj = types.reshape(lz, ly, lx) > 0
+    t = types.reshape(lz, ly, lx)
+    code = np.zeros(t.shape, dtype=int)
+    for axis, w in ((2, 1), (1, 4), (0, 16)):
+        up = np.roll(obj, -1, axis=axis)
+        down = np.roll(obj, 1, axis=axis)
+        code += w * np.where(down, 2, np.where(up, 1, 0))
+    return set(np.unique(code[t == -1]).tolist())
+
+
+@pytest.mark.parametrize("semantics", [0, 1])
+@pytest.mark.parametrize("params", [MULTI, ALL], ids=["L1-L3", "L1-L6"])
+def test_forces_cover_every_boundary_case(semantics, params):
+    """Two spheres (radius 5.6 homeotropic, 6.7 degenerate planar) whose surface
+    sites take all 26 object-neighbour cases of the boundary stencils, and a
+    one-site object; bulk sites near them need one-sided neighbour derivatives
+    (the boundary pass's case 0; 519 here): per-site forces against the oracle,
+    and FIRE iterations."""
+    shape = (36, 26, 24)
+    o = po.OracleSystem(*shape)
+    g = qmin.Lattice(*shape)
+    for sys_ in (o, g):
+        sys_.random_init(77, S0)
+        sys_.sphere((10.3, 12.6, 11.8), 5.6, 0, 5.3, S0)
+        sys_.sphere((26.6, 13.2, 12.4), 6.7, 1, 3.1, S0)
+        sys_.sphere((18.5, 4.5, 20.5), 0.9, 0, 8.0, S0)
+    types = o.types()
+    assert np.array_equal(g.types(), types)
+    cases = _boundary_cases(types, shape)
+    assert cases == set(range(64)) - {0} - {c for c in range(64) if c % 4 == 3 or (c // 4) % 4 == 3 or c // 16 >= 3}
+    assert len(cases) == 26
+    # bulk sites next to a boundary site that has an object beside it (across
+    # the bulk site's line to it): their neighbour derivatives are one-sided
+    t3 = types.reshape(shape[2], shape[1], shape[0])
+    obj = t3 > 0
+    one_sided = np.zeros(t3.shape, bool)
+    for ax in range(3):
+        for sgn in (1, -1):
+            nb_is_surface = np.roll(t3, -sgn, axis=ax) == -1
+            beside = np.zeros(t3.shape, bool)
+            for ax2 in range(3):
+                if ax2 != ax:
+                    for s2 in (1, -1):
+                        beside |= np.roll(np.roll(obj, -sgn, axis=ax), -s2, axis=ax2)
+            one_sided |= (t3 == 0) & nb_is_surface & beside
+    assert one_sided.sum() >= 12
+    o.set_params(A, B, C, *params, semantics=semantics)
+    f = qmin.LandauDeGennesLC(A, B, C, *params, semantics=semantics)
+    assert_forces_close(f.compute_forces(g), o.forces())
+    it_ref, tr_ref = o.fire(max_iter=15)
+    m = qmin.FIRE(max_iter=15)
+    m.set_trace(16)
+    m.minimize(f, g)
+    np.testing.assert_allclose(m.trace()[:, 0], tr_ref[:, 0], rtol=1e-10)
+    np.testing.assert_allclose(g.q(), o.q(), rtol=0, atol=1e-11)
+
+
 def test_forces_with_field():
     o, g, f = build_pair((20, 16, 18), ALL, 0, True, efield=(0.0, 0.0, 0.5))
     assert_forces_close(f.compute_forces(g), o.forces())
