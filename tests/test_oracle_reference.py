@@ -652,3 +652,22 @@ def test_l2_l3_l6_stencils_are_not_covariant_on_smooth_fields():
     assert mismatch["L1"] < 1e-12 and mismatch["L4"] < 1e-12, mismatch
     for name in ("L2", "L3", "L6"):
         assert 0.05 < mismatch[name] < 0.5, mismatch
+
+
+@live_ref
+@pytest.mark.parametrize("kind", [0, 1, 2])
+def test_defect_measures_match_reference(phase, kind):
+    """computeDefectMeasures (qTensorLatticeModel.cpp:95-133) of the compiled
+    reference model for all three kinds (largest eigenvalue, determinant,
+    degree of biaxiality) on a random biaxial field (the biaxiality measure is
+    zero on the uniaxial seeded start): the oracle's equal them on every site."""
+    shape = (7, 6, 5)
+    n = shape[0] * shape[1] * shape[2]
+    q = np.random.default_rng(5).normal(0.0, 0.3, size=(n, 5))
+    r = po.RefModel(*shape)
+    r.set_q(q)
+    o = po.OracleSystem(*shape)
+    o.set_q(q)
+    want = r.defects(kind)
+    assert np.abs(want).max() > 0
+    np.testing.assert_allclose(o.defects(kind), want, rtol=1e-12, atol=1e-15 * np.abs(want).max())
