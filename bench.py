@@ -319,11 +319,42 @@ def ensemble_throughput(qmin, count=1024, fc=1e-7, max_iter=10000, sequential_sa
     return {"lattices": count, "sites": int(sum(sites)), "dtype": "f32 (FP64 reductions)",
             "forceMax_below": fc, "max_iterations": max_iter, "seconds": t2 - t1, "build_seconds": build_s,
             "site_updates_per_s": updates / (t2 - t1), "converged": int(sum(1 for f in fms if f <= fc)),
-            "autotune": {"tile": ["32x8", "64x4", "32x16", "64x8"][tile], "zchunk": zchunk, "pass_ms": pass_ms,
-                         "seconds": t1 - ta},
+            "autotune": {"tile": "per lattice" if tile == qmin.TILE_PER_LATTICE else ["32x8", "64x4", "32x16", "64x8"][tile],
+                         "zchunk": zchunk, "pass_ms": pass_ms, "seconds": t1 - ta},
             "iterations_min_mean_max": [int(min(its)), float(sum(its)) / count, int(max(its))],
             "one_by_one": {"lattices": sequential_sample, "site_updates_per_s": seq_updates / (t4 - t3),
                            "same_iterations": [int(i) for i in its[:sequential_sample]] == seq_its}}
+
+
+def full_elasticity_iteration_times(qmin, L=384, iterations=10):
+    """Per-iteration wall time of FIRE, Adam, Nesterov and online L-BFGS (memory
+    8) on the full-elasticity workload (qmin.workloads.full_elasticity: L1..L4,
+    L6, E field, four spheres) in FP64, each after two warm-up iterations, on one
+    lattice in turn (a bounded leg of the default run; --full-elasticity runs each
+    to tolerance)."""
+    from qmin import workloads
+    lat, f = workloads.full_elasticity(qmin, L, qmin.QM_FP64)
+    runs = (("FIRE", qmin.FIRE(dt=5e-4, dt_max=2e-2, fc=1e-12, max_iter=10 ** 9)),
+            ("Adam", qmin.Adam(dt=1e-4, fc=1e-12, max_iter=10 ** 9)),
+            ("Nesterov", qmin.Nesterov(dt=1e-4, mu=0.01, fc=1e-12, max_iter=10 ** 9)),
+            ("LoLBFGS(m=8)", qmin.LoLBFGS(m=8, dt=1e-4, fc=1e-12, max_iter=10 ** 9)))
+    out = {"size": L, "dtype": "f64", "iterations": iterations}
+    for name, m in runs:
+        if name == "FIRE":
+            m.run_iterations(f, lat, 2)
+            t0 = time.perf_counter()
+            m.run_iterations(f, lat, iterations)
+        else:
+            m.set_max_iterations(2)
+            m.minimize(f, lat)
+            m.set_max_iterations(2 + iterations)
+            t0 = time.perf_counter()
+            m.minimize(f, lat)
+        ms = (time.perf_counter() - t0) / iterations * 1e3
+        out[name] = {"ms_per_iteration": ms, "site_updates_per_s": lat.n / ms * 1e3}
+        print("full elasticity %d^3 FP64, %s: %.3f ms per iteration" % (L, name, ms), file=sys.stderr, flush=True)
+    del lat
+    return out
 
 
 # Minimizers of the full-elasticity workload: (name, class, parameters, iteration
@@ -563,8 +594,13 @@ def main():
         out["time_to_tolerance"]["warm_start_heldout_128"] = warm_start_heldout(qmin)
     if args.ensemble:
         out["ensemble"] = ensemble_throughput(qmin)
+    elif not args.no_ttt:
+        # bounded legs of the two other workloads, so that the default run records them too
+        out["ensemble_256"] = ensemble_throughput(qmin, count=256, max_iter=512, sequential_sample=8)
     if args.full_elasticity:
         out["full_elasticity"] = full_elasticity(qmin, fc=args.ttt_fc)
+    elif not args.no_ttt:
+        out["full_elasticity_iteration_times"] = full_elasticity_iteration_times(qmin)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))
