@@ -1590,7 +1590,10 @@ static int fire_drive(qm_fire* m, qm_ldg* f, qm_lattice* L, int maxIter, bool to
     int rc = L->precision == QM_FP64 ? run_pass<double>(L, f, kForce, fp, false, partials, false, streamFire)
                                      : run_pass<float>(L, f, kForce, fp, false, partials, false, streamFire);
     if (rc) return rc;
-    const int batch = L->n >= (1 << 22) ? 16 : 64;
+    // With a tolerance the host looks at the control block every batch (the
+    // device skips the passes after the stop anyway); a run of known length is
+    // queued in batches of up to 256 iterations.
+    const int batch = !tolerance ? 256 : (L->n >= (1 << 22) ? 16 : 64);
     // without a tolerance the run's last pass is known: it stores F as well
     bool lastStoresF = false;
     while (true) {
