@@ -74,11 +74,13 @@ constexpr bool stream_fits() {
 }
 
 // minimum waves per SIMD asked of the compiler: for the 32 x 16 tile (12
-// waves), two work-groups per CU in FP32 (80 registers), one in FP64, whose
-// rings fill the LDS; the other tiles take what their registers give
+// waves) one work-group per CU.  In FP64 its rings fill the LDS; in FP32 two
+// work-groups would fit the LDS but not 80 registers, which spill (0.654 against
+// 0.540 ms per pass at 256^3 with one).  The other tiles take what their
+// registers give.
 template <typename R, int TX, int TY>
 constexpr int stream_waves() {
-    return (TX == 32 && TY == 16) ? (sizeof(R) == 4 ? 6 : 3) : 1;
+    return (TX == 32 && TY == 16) ? 3 : 1;
 }
 
 }  // namespace
